@@ -1,0 +1,238 @@
+"""Regular-force benchmark of the MI355X engine (one JSON line on stdout).
+
+Workload: King W0=6 cluster, N = 131072 bodies, Kroupa IMF (0.08-150 Msun,
+total mass 1), Henon units; neighbour radii with 48 distance neighbours
+(~55 with the predicted-distance criterion), STEPR as FPOLY0 sets it.
+One step = one regular block over ALL bodies: the device predictor
+(gpunb_predict_) followed by the regular force of all N i-particles against
+all N j-particles (gpunb_regf_dev: inputs and outputs resident in HBM),
+i.e. N^2 = 1.72e10 pair interactions per step.
+
+value = pair interactions / s over the timed steps (wall clock between two
+device synchronisations).  roofline: the pair kernel (k_regf_pass1), timed
+with HIP events on the library stream around every launch in the timed
+region; achieved = 60 flops per pair (the reference's convention,
+gpunb.gpu.cu:811) x pairs per launch / mean launch time, against the MI355X
+FP32 peak.  cpu_baseline: the reference's own AVX regular force
+(src/Main/reg.avx.cpp, compiled from source under oracle/_ref) on the same
+j set, i in chunks of 1024 as the Fortran host calls it (util_gpu.F:33-37),
+timed on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP32_PEAK_TFLOPS = 157.3       # MI355X FP32 (vector and matrix) peak, AMD spec
+FLOPS_PER_PAIR = 60            # reference convention (gpunb.gpu.cu:811)
+N_BODIES = 131072
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def workload(n=N_BODIES, seed=7):
+    from nbody6ppgpu_amd import ics
+    c = ics.king(n, 6.0, seed=seed, m=ics.kroupa(n, seed=seed + 1))
+    rs, stepr = ics.neighbour_radii(c, k=48)
+    return c, rs * rs, stepr
+
+
+def load_traffic():
+    """HBM bytes per pair-kernel launch from the committed rocprofv3 PMC
+    summary (profiles/regf_pass1_king131072_traffic.json), if present."""
+    p = os.path.join(ROOT, "profiles", "regf_pass1_king131072_traffic.json")
+    if os.path.exists(p):
+        try:
+            with open(p) as f:
+                return json.load(f).get("hbm_bytes_per_launch")
+        except (OSError, ValueError):
+            return None
+    return None
+
+
+def cpu_baseline(c, h2, stepr, budget_s=12.0):
+    """Reference AVX regular force (oracle/_ref) on the same j set."""
+    from oracle import ref
+    n = c.n
+    lmax, nnbmax = 601, 550
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    threads = min(threads, 32)      # reg.avx.cpp:7,103 asserts <= 32 threads
+    if ref.available("reg_avx"):
+        R = ref.RegLib("reg_avx", n + 10)
+        R.send(c.m, c.x, c.v)
+        done = 0
+        t0 = time.perf_counter()
+        while done < n and time.perf_counter() - t0 < budget_s:
+            sl = slice(done, min(done + 1024, n))
+            R.regf(h2[sl], stepr[sl], c.x[sl], c.v[sl], lmax, nnbmax, 0)
+            done = sl.stop
+        dt = time.perf_counter() - t0
+        R.close()
+        return {"value": done * n / dt, "unit": "pair interactions/s",
+                "cores": threads, "kind": "reference",
+                "sample": f"reference reg.avx.cpp gpunb_regf_, {done} of {n} i "
+                          f"in 1024-i calls against all {n} j, "
+                          f"OMP_NUM_THREADS={threads}, {dt:.1f} s"}
+    from oracle import oracle
+    ni = 256
+    t0 = time.perf_counter()
+    oracle.regf(c.m, c.x, c.v, h2[:ni], stepr[:ni], c.x[:ni], c.v[:ni], lmax,
+                nnbmax, 0)
+    dt = time.perf_counter() - t0
+    return {"value": ni * n / dt, "unit": "pair interactions/s", "cores": 1,
+            "kind": "port", "sample": f"FP64 oracle, {ni} i against {n} j"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true",
+                    help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        sys.exit("bench.py runs on one GPU only: this build measures the "
+                 "regular-force engine on a single MI355X (--gpus 1).")
+
+    from nbody6ppgpu_amd import gpunb
+    from nbody6ppgpu_amd.hiprt import DeviceArray, Event, device_synchronize
+
+    log("bench: building workload (King W0=6, N=131072, Kroupa IMF)")
+    c, h2, stepr = workload()
+    n = c.n
+    nnbmax = 550
+    gpunb.gpunb_open(n + 10)
+    gpunb.gpunb_jpset(np.arange(1, n + 1), c.x, c.v, np.zeros((n, 3)),
+                      np.zeros((n, 3)), c.m, np.zeros(n))
+    d = {k: DeviceArray.from_numpy(a) for k, a in
+         (("x", c.x), ("v", c.v), ("h2", h2), ("dtr", stepr))}
+    acc = DeviceArray((n, 3), np.float64)
+    jrk = DeviceArray((n, 3), np.float64)
+    pot = DeviceArray((n,), np.float64)
+    nnb = DeviceArray((n,), np.int32)
+    nnid = DeviceArray((n,), np.int32)
+    off = DeviceArray((n,), np.int32)
+    cap = n * nnbmax
+    lst = DeviceArray((cap,), np.int32)
+    tot = DeviceArray((1,), np.int32)
+    lib = gpunb.load()
+    stream = gpunb.gpunb_stream()
+
+    def step():
+        gpunb.gpunb_predict(0.0)
+        rc = lib.gpunb_regf_dev(n, d["h2"].ptr, d["dtr"].ptr, d["x"].ptr,
+                                d["v"].ptr, acc.ptr, jrk.ptr, pot.ptr, nnbmax,
+                                0, nnb.ptr, nnid.ptr, lst.ptr, off.ptr,
+                                tot.ptr, cap)
+        if rc != 0:
+            gpunb._check()
+
+    log(f"bench: warmup {args.warmup}")
+    for _ in range(args.warmup):
+        step()
+    gpunb.gpunb_sync()
+    counts = nnb.numpy()
+    assert np.all(counts >= 1), "every body must list itself"
+    nnb_mean = float(counts.mean())
+
+    log(f"bench: timing {args.steps} steps")
+    gpunb.gpunb_timing(True)
+    e0, e1 = Event(), Event()
+    device_synchronize()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(args.steps):
+        step()
+    e1.record(stream)
+    gpunb.gpunb_sync()
+    device_synchronize()
+    t1 = time.perf_counter()
+    k_ms, k_launches, k_pairs = gpunb.gpunb_timing_read()
+    gpunb.gpunb_timing(False)
+    ev_ms = e0.elapsed_ms(e1)
+    wall_ms = (t1 - t0) * 1e3
+    ms_per_step = wall_ms / args.steps
+    pairs_per_step = float(n) * n
+    value = pairs_per_step / (ms_per_step * 1e-3)
+
+    avg_launch_s = k_ms * 1e-3 / max(k_launches, 1)
+    pairs_per_launch = k_pairs / max(k_launches, 1)
+    achieved = FLOPS_PER_PAIR * pairs_per_launch / avg_launch_s / 1e12
+    log(f"bench: {ms_per_step:.3f} ms/step (events {ev_ms / args.steps:.3f}), "
+        f"pair kernel {k_launches} launches, mean {avg_launch_s * 1e3:.3f} ms")
+
+    # Drop-in ABI path with host arrays (PCIe-inclusive), i in chunks of
+    # 1024 as the Fortran host calls it; reported next to value, never as it.
+    gpunb.gpunb_send(c.m, c.x, c.v)
+    nchunk = 32
+    t2 = time.perf_counter()
+    for q in range(nchunk):
+        sl = slice(q * 1024, (q + 1) * 1024)
+        gpunb.gpunb_regf(h2[sl], stepr[sl], c.x[sl], c.v[sl], nnbmax + 1,
+                         nnbmax, 0)
+    abi_rate = nchunk * 1024 * n / (time.perf_counter() - t2)
+    gpunb.gpunb_close()
+
+    cpu = None
+    if not args.no_cpu:
+        log("bench: CPU baseline (reference AVX path)")
+        cpu = cpu_baseline(c, h2, stepr)
+
+    traffic = load_traffic()
+    out = {
+        "metric": "regular-force pair interactions/s at N=128k, % FP32 peak; "
+                  "dE/E per t_NB",
+        "value": value,
+        "unit": "pair interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "King W0=6, N=131072, Kroupa IMF 0.08-150 Msun; one "
+                        "regular block over all N bodies (device predictor + "
+                        "regular force of N i vs N j), inputs resident in HBM",
+            "n": n, "ni_per_step": n, "pairs_per_step": pairs_per_step,
+            "nnb_mean": nnb_mean, "nnbmax": nnbmax,
+            "precision": "FP64 differences, FP64 neighbour predicate and "
+                         "accumulation; FP32 pair arithmetic",
+        },
+        "roofline": {
+            "bound": "mfma",
+            "compute_unit": "FP32 VALU (pairwise 1/r^3 sums are not a dense "
+                            "contraction); peak = MI355X FP32 peak",
+            "kernel": "k_regf_pass1",
+            "achieved": achieved,
+            "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / FP32_PEAK_TFLOPS,
+            "traffic": traffic,
+            "flops_per_pair": FLOPS_PER_PAIR,
+            "pairs_per_launch": pairs_per_launch,
+            "mean_launch_ms": avg_launch_s * 1e3,
+            "launches": k_launches,
+        },
+        "cpu_baseline": cpu,
+        "abi_ni1024_pairs_per_s": abi_rate,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

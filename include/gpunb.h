@@ -1,0 +1,191 @@
+/*
+ * gpunb.h -- C-ABI of the MI355X regular-force engine for NBODY6++GPU.
+ *
+ * The first block is the drop-in boundary: the exact symbols, argument order
+ * and by-reference convention of the reference's regular-force libraries, so
+ * the unchanged Fortran host (implicit interfaces, trailing underscore, every
+ * argument by reference) links against this library in place of the CUDA
+ * `gpunb` + `gpupot` objects.  Every entry point cites the reference function
+ * it replaces (paths relative to the reference tree).
+ *
+ * Array conventions (Fortran column-major == C row-major of the transposed
+ * shape):
+ *   Fortran X(3,N)        == C double x[N][3]
+ *   Fortran LISTGP(LMAX,NI)== C int    list[NI][LMAX]
+ * Neighbour and nearest-neighbour indices are 0-based, relative to the first
+ * j-particle passed to gpunb_send_ (the caller adds IFIRST, util_gpu.F:105).
+ *
+ * Error behaviour: like the reference (assert/abort on precondition or device
+ * errors, gpunb.gpu.cu:703-709, cuda_pointer.h:20-42) the library prints a
+ * line "gpunb: ..." on stderr and aborts.  A caller that prefers to handle
+ * errors itself calls gpunb_set_abort_(&zero) once; failing calls then return
+ * with their outputs untouched and gpunb_last_error() reports the message.
+ * Neighbour-list overflow is not an error: it is reported in-band as a
+ * negative count, as in the reference.
+ *
+ * Threading: calls come from one host thread (util_gpu.F:60 sits between
+ * OpenMP loops); the library is not re-entrant, as the reference is not.
+ */
+#ifndef GPUNB_MI355X_H
+#define GPUNB_MI355X_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Drop-in boundary (reference ABI)                                          */
+/* ------------------------------------------------------------------------ */
+
+/* Select the device: first entry of the GPU_LIST environment variable if set,
+ * otherwise irank % device_count; prints the device name once.  Idempotent.
+ * Replaces GPUNB_devinit / gpunb_devinit_ (src/Main/gpunb.gpu.cu:557-590,822-824). */
+void gpunb_devinit_(int *irank);
+
+/* Open the library for up to *nbmax j-particles.  A second open without close
+ * prints "gpunb: it is already open" and returns.
+ * Replaces GPUNB_open / gpunb_open_ (src/Main/gpunb.gpu.cu:592-634,825-827). */
+void gpunb_open_(int *nbmax, int *irank);
+
+/* Release device and pinned buffers.  A second close prints a warning.
+ * Replaces GPUNB_close / gpunb_close_ (src/Main/gpunb.gpu.cu:636-668,828-830). */
+void gpunb_close_(void);
+
+/* Upload *nj predicted j-particles (mass, position, velocity; FP64).  They stay
+ * resident in device memory for any number of following gpunb_regf_ calls.
+ * Requires *nj <= nbmax of the open.  NaN inputs are an error.
+ * Replaces GPUNB_send / gpunb_send_ (src/Main/gpunb.gpu.cu:670-688,831-833). */
+void gpunb_send_(int *nj, double *mj, double (*xj)[3], double (*vj)[3]);
+
+/* Regular force of *ni i-particles against all j-particles last sent.
+ *   h2[i]  squared neighbour radius RS^2 (RS^2/<m> when *m_flag != 0)
+ *   dtr[i] regular step, used by the predicted-distance criterion
+ * Outputs, for every i (FP64): far-field acceleration acc[i][3], jerk
+ * jrk[i][3] and potential pot[i] = sum over non-neighbours of m_j/r_ij
+ * (positive), and the neighbour list in list[i*lmax + 0 .. nnb]:
+ *   list[i*lmax]      = nnb             if 0 <= nnb <= *nnbmax
+ *                     = -nnb            if nnb > *nnbmax (overflow; rest undefined)
+ *   list[i*lmax+1..]  = j indices, strictly ascending, self included
+ * j is a neighbour of i iff min(r2, r2p) < h2[i]   (m_flag == 0)
+ *                      or   min(r2, r2p) < m_j*h2[i] (m_flag != 0),
+ * with dx = x_j - x_i, dv = v_j - v_i, r2 = |dx|^2 and r2p = |dx + dtr*dv|^2,
+ * all evaluated in FP64 exactly as written (no fused multiply-add); pairs
+ * with r2 == 0 never contribute to acc/jrk/pot.
+ * Any *ni >= 0 is accepted (the reference asserts 0 < ni <= 1024 and the
+ * Fortran host chunks at 1024, util_gpu.F:33-37); *lmax must be > *nnbmax.
+ * Replaces GPUNB_regf / gpunb_regf_ (src/Main/gpunb.gpu.cu:690-796,834-838;
+ * CPU twin src/Main/reg.avx.cpp:185-331). */
+void gpunb_regf_(int *ni, double *h2, double *dtr, double (*xi)[3],
+                 double (*vi)[3], double (*acc)[3], double (*jrk)[3],
+                 double *pot, int *lmax, int *nnbmax, int *list, int *m_flag);
+
+/* Print the per-interval profile line (send/grav/nb/out seconds and Gflops
+ * at the reference's 60 flops per pair interaction) and reset the counters.
+ * Replaces GPUNB_profile / gpunb_profile_ (src/Main/gpunb.gpu.cu:799-819,839-841). */
+void gpunb_profile_(int *irank);
+
+/* Potential phi_k = sum over j in [0,*n) with r2 > 0 of m_j / r_jk for the
+ * particles k = *istart-1 .. *istart-2+*ni (1-based *istart, as energy.F:38
+ * calls it); pot[0..*ni-1] receives them (positive values).  Independent of
+ * the open/send state (it calls gpunb_devinit_ itself).
+ * Replaces gpupot / gpupot_ (src/Main/gpupot.gpu.cu:61-127; CPU twin
+ * src/Main/pot.avx.cpp:72-159). */
+void gpupot_(int *irank, int *istart, int *ni, int *n, double *m,
+             double (*x)[3], double *pot);
+
+/* ------------------------------------------------------------------------ */
+/* Additive entry points (no reference symbol is changed)                    */
+/* ------------------------------------------------------------------------ */
+
+/* gpunb_regf_ plus the nearest-neighbour index: nnid[i] = the listed j with
+ * r2 > 0 of smallest FP64 r2 (ties: smaller j), -1 if there is none or the
+ * list overflowed.  Semantics of IMINR in src/Main/nbint.F:29-47 (strict <,
+ * first in ascending list order); the reference computes it on the host or in
+ * the irregular library (src/Main/irr.avx.cpp:317-320), never in regf. */
+void gpunb_regfnn_(int *ni, double *h2, double *dtr, double (*xi)[3],
+                   double (*vi)[3], double (*acc)[3], double (*jrk)[3],
+                   double *pot, int *lmax, int *nnbmax, int *list,
+                   int *m_flag, int *nnid);
+
+/* Device-resident j predictor, part 1: store the Hermite state of *n
+ * j-particles at the 1-based j slots addr[k] (slot 1 = first j):
+ * x0, v0 (X0, X0DOT), f = a/2 (F), fdot = jerk/6 (FDOT), mass and t0.
+ * Mirrors the per-particle irr_simd_set_jp_ (src/Main/irr.avx.cpp:582-592)
+ * as one bulk call.  Slots above the current j count extend it. */
+void gpunb_jpset_(int *n, int *addr, double (*x0)[3], double (*v0)[3],
+                  double (*f)[3], double (*fdot)[3], double *mass,
+                  double *t0);
+
+/* Device-resident j predictor, part 2: predict every stored j-particle to
+ * *time and make the result the j set used by gpunb_regf_ (FP64):
+ *   s = time - t0;  x = ((fdot*s + f)*s + v0)*s + x0;
+ *   v = (fdot*1.5s + f)*2s + v0
+ * Replaces the host XBPREDALL + GPUNB_SEND pair (src/Main/xbpredall.f:15-27,
+ * src/Main/intgrt.F:515-521,912-918) for a host that keeps the j state on
+ * the device; the KS pair resolution of xbpredall.f:29-35 stays on the host. */
+void gpunb_predict_(double *time);
+
+/* Copy the current (predicted or sent) j positions/velocities back to the
+ * host for the first *n j-particles. */
+void gpunb_jpget_(int *n, double (*x)[3], double (*v)[3]);
+
+/* Error handling switch: *on != 0 (default) aborts on error like the
+ * reference; *on == 0 returns instead and records the message. */
+void gpunb_set_abort_(int *on);
+
+/* Last error message ("" if none) and clear it.  Returns its length. */
+int gpunb_last_error(char *buf, int len);
+
+/* ------------------------------------------------------------------------ */
+/* Device-resident entry points (plain device pointers, asynchronous on the  */
+/* library stream).  For callers that keep i data and results in HBM.        */
+/* ------------------------------------------------------------------------ */
+
+/* Regular force with every input and output in device memory, enqueued on
+ * the library stream and not synchronised.
+ *   d_xi, d_vi : double[ni][3]     d_h2, d_dtr : double[ni]
+ *   d_acc, d_jrk : double[ni][3]   d_pot : double[ni]
+ *   d_nnb : int[ni]  (count, or -count on overflow, as list[i*lmax])
+ *   d_nnid: int[ni]  (may be NULL)
+ *   d_list: int[list_cap] compact neighbour lists; d_off[i] = start of i's
+ *           list in d_list (-1 on overflow); d_total: int[1], set to the
+ *           number of list entries written (the order of the per-i blocks in
+ *           d_list is not fixed, their content is).
+ * Returns 0, or -1 with gpunb_last_error() set (including list_cap too small,
+ * which is reported after the fact by gpunb_sync(): the lists are then
+ * incomplete). */
+int gpunb_regf_dev(int ni, const double *d_h2, const double *d_dtr,
+                   const double *d_xi, const double *d_vi, double *d_acc,
+                   double *d_jrk, double *d_pot, int nnbmax, int m_flag,
+                   int *d_nnb, int *d_nnid, int *d_list, int *d_off,
+                   int *d_total, long long list_cap);
+
+/* Wait for all work on the library stream; returns 0 or -1 (error set). */
+int gpunb_sync(void);
+
+/* The library's HIP stream (as void*), for callers that time or order work
+ * against it. */
+void *gpunb_stream(void);
+
+/* Library build/version string. */
+const char *gpunb_version(void);
+
+/* Work decomposition of the regular-force kernel (autotuner hook):
+ * group = pairs per branch-free group of the inner loop (2 or 4; 0 = default),
+ * rounds = resident-wave capacity multiples per launch (0 = default).
+ * Returns 0, or -1 with gpunb_last_error() set. */
+int gpunb_set_tuning(int group, double rounds);
+
+/* Kernel timing: with enable != 0 the library records HIP events on its own
+ * stream around every launch of the regular-force pair kernel; a read
+ * synchronises and returns the summed kernel time (ms), the number of
+ * launches and the pair interactions they computed, then resets. */
+int gpunb_timing(int enable);
+int gpunb_timing_read(double *ms_total, long long *launches,
+                      long long *pairs);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPUNB_MI355X_H */

@@ -1,0 +1,883 @@
+// gpunb.hip -- host side of the MI355X regular-force engine: the C-ABI of
+// include/gpunb.h, device state, staging buffers and the work decomposition.
+//
+// One library stream (default priority) carries every copy and kernel.  The
+// ABI entry points stage host arrays through pinned buffers, enqueue the
+// kernels and synchronise once per call; the *_dev entry points work on
+// caller-owned device memory and do not synchronise.
+#include "gpunb_internal.h"
+#include "gpunb.h"
+
+#include <hip/hip_runtime.h>
+#include <sys/time.h>
+#include <unistd.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace gpunb {
+namespace {
+
+struct Error : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define HIPCHK(expr)                                                         \
+    do {                                                                     \
+        hipError_t rc_ = (expr);                                             \
+        if (rc_ != hipSuccess)                                               \
+            throw Error(std::string(#expr) + " failed: " +                   \
+                        hipGetErrorString(rc_));                             \
+    } while (0)
+
+#define REQUIRE(cond, msg)                     \
+    do {                                       \
+        if (!(cond)) throw Error(msg);         \
+    } while (0)
+
+double wtime() {
+    struct timeval tv;
+    gettimeofday(&tv, nullptr);
+    return tv.tv_sec + 1.e-6 * tv.tv_usec;
+}
+
+bool env_flag(const char *name, bool dflt) {
+    const char *v = getenv(name);
+    if (!v || !*v) return dflt;
+    return !(v[0] == '0' || v[0] == 'n' || v[0] == 'N' || v[0] == 'f' ||
+             v[0] == 'F');
+}
+
+// Device buffer that only grows.
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    void ensure(size_t need) {
+        if (need <= n) return;
+        release();
+        size_t bytes = std::max<size_t>(need, 1) * sizeof(T);
+        HIPCHK(hipMalloc((void **)&p, bytes));
+        n = need;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+// Pinned host buffer that only grows.
+template <class T>
+struct HostBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    void ensure(size_t need) {
+        if (need <= n) return;
+        release();
+        size_t bytes = std::max<size_t>(need, 1) * sizeof(T);
+        HIPCHK(hipHostMalloc((void **)&p, bytes, hipHostMallocDefault));
+        n = need;
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+struct State {
+    bool devinit = false;
+    bool open = false;
+    int dev = 0;
+    int ndev = 0;
+    int rank = 0;
+    hipStream_t st = nullptr;
+    hipEvent_t ev_send = nullptr, ev_jp = nullptr;
+    bool ev_send_live = false, ev_jp_live = false;
+
+    int nbmax = 0;
+    int nj = 0;       // current j count used by regf
+    int njp = 0;      // j slots covered by the predictor state
+    DevBuf<JRec> jp;
+    DevBuf<JStats> jst;
+    DevBuf<unsigned> err;
+    HostBuf<unsigned> h_err;
+
+    HostBuf<double> h_send;
+    DevBuf<double> d_send;
+
+    DevBuf<double> x0, v0, f, fd, t0, mass;
+    HostBuf<double> h_jp;
+    DevBuf<double> d_jp;
+    HostBuf<int> h_addr;
+    DevBuf<int> d_addr;
+
+    HostBuf<double> h_in;
+    DevBuf<double> d_in;
+    DevBuf<double> part;
+    DevBuf<int> cnt;
+    DevBuf<uint16_t> seg;
+    DevBuf<double> d_out;
+    HostBuf<double> h_out;
+    DevBuf<int> d_iout;
+    HostBuf<int> h_iout;
+    DevBuf<int> d_list;
+    HostBuf<int> h_list;
+    long long list_est = 0;
+
+    DevBuf<PRec> prec;
+    DevBuf<double> d_pin, d_ppart, d_pout;
+    HostBuf<double> h_pin, h_pout;
+
+    double time_send = 0, time_grav = 0, time_nb = 0, time_out = 0;
+    long long numInter = 0;
+    int icall = 0, ini = 0, isend = 0;
+
+    // Kernel timing (gpunb_timing): events around every pass-1 launch.
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    int ev_used = 0;
+    std::vector<std::pair<int, int>> ev_pairs;   // indices into ev_pool
+    std::vector<long long> ev_pairs_work;        // pair interactions per launch
+
+    int resident[2][2] = {{0, 0}, {0, 0}};
+    int pot_resident = 0;
+    int group = 0;          // 0: choose automatically
+    double rounds = 0.0;    // 0: default
+};
+
+State S;
+bool g_abort = true;
+bool g_abort_set = false;
+std::string g_last_error;
+
+bool quiet() { return env_flag("GPUNB_QUIET", false); }
+
+void report(const char *fn, const char *msg) {
+    g_last_error = std::string(fn) + ": " + msg;
+    fprintf(stderr, "gpunb: %s\n", g_last_error.c_str());
+    fflush(stderr);
+    bool ab = g_abort_set ? g_abort : env_flag("GPUNB_ABORT", true);
+    if (ab) abort();
+}
+
+template <class F>
+int guarded(const char *fn, F &&f) {
+    try {
+        f();
+        return 0;
+    } catch (const std::exception &e) {
+        report(fn, e.what());
+        return -1;
+    }
+}
+
+int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
+int round_up(int a, int b) { return cdiv(a, b) * b; }
+
+void devinit(int irank) {
+    if (S.devinit) return;
+    HIPCHK(hipGetDeviceCount(&S.ndev));
+    REQUIRE(S.ndev > 0, "no HIP device visible");
+    int dev = irank % S.ndev;
+    const char *gl = getenv("GPU_LIST");
+    if (gl && *gl) {
+        // First entry of the space-separated list (gpunb.gpu.cu:562-571);
+        // parsed from a copy so the environment is left untouched.
+        std::string first(gl);
+        size_t a = first.find_first_not_of(' ');
+        REQUIRE(a != std::string::npos, "GPU_LIST is empty");
+        dev = atoi(first.c_str() + a);
+        REQUIRE(dev >= 0 && dev < S.ndev, "GPU_LIST names a missing device");
+    }
+    S.dev = dev;
+    S.rank = irank;
+    HIPCHK(hipSetDevice(dev));
+    HIPCHK(hipStreamCreateWithFlags(&S.st, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&S.ev_send, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&S.ev_jp, hipEventDisableTiming));
+    S.jst.ensure(1);
+    S.err.ensure(1);
+    S.h_err.ensure(1);
+    HIPCHK(hipMemsetAsync(S.err.p, 0, sizeof(unsigned), S.st));
+    launch_stats_reset(S.st, S.jst.p, nullptr);
+    HIPCHK(hipStreamSynchronize(S.st));
+    if (!quiet()) {
+        hipDeviceProp_t prop;
+        HIPCHK(hipGetDeviceProperties(&prop, dev));
+        char host[150];
+        memset(host, 0, sizeof(host));
+        gethostname(host, sizeof(host) - 1);
+        fprintf(stderr,
+                "# gpunb (MI355X) init - rank: %d; host %s; devices %d; "
+                "device %d %s (%s, %d CUs)\n",
+                irank, host, S.ndev, dev, prop.name, prop.gcnArchName,
+                prop.multiProcessorCount);
+    }
+    S.devinit = true;
+}
+
+void set_device() { HIPCHK(hipSetDevice(S.dev)); }
+
+// The device error word accumulates until the next synchronisation point
+// (an error raised by an asynchronous send is reported by the following
+// regf); err_check clears it when it reports.
+void err_fetch() {
+    HIPCHK(hipMemcpyAsync(S.h_err.p, S.err.p, sizeof(unsigned),
+                          hipMemcpyDeviceToHost, S.st));
+}
+void err_check() {
+    unsigned e = *S.h_err.p;
+    if (!e) return;
+    *S.h_err.p = 0;
+    HIPCHK(hipMemsetAsync(S.err.p, 0, sizeof(unsigned), S.st));
+    HIPCHK(hipStreamSynchronize(S.st));
+    std::string m;
+    if (e & ERR_NAN_J) m += "NaN in j-particle data; ";
+    if (e & ERR_NAN_I) m += "NaN in i-particle data; ";
+    if (e & ERR_LIST_CAP) m += "neighbour list buffer too small; ";
+    throw Error(m);
+}
+
+int pick_group() {
+    if (S.group) return S.group;
+    const char *g = getenv("GPUNB_REGF_GROUP");
+    if (g && atoi(g) == 2) return 2;
+    if (g && atoi(g) == 4) return 4;
+    return 4;
+}
+
+double pick_rounds() {
+    if (S.rounds > 0) return S.rounds;
+    const char *r = getenv("GPUNB_WAVE_ROUNDS");
+    if (r && atof(r) > 0) return atof(r);
+    return 2.0;
+}
+
+int resident_waves(int m_flag, int group) {
+    int &r = S.resident[m_flag ? 1 : 0][group == 2 ? 0 : 1];
+    if (!r) r = regf_resident_waves(m_flag, group);
+    if (r <= 0) r = 256 * 8;
+    return r;
+}
+
+struct Decomp {
+    int nslice;
+    int cap;
+};
+
+// j slices per i-wavefront: enough waves to fill the resident capacity
+// `rounds` times; slices hold whole tiles, at most 1024 tiles (uint16 local
+// neighbour indices).
+Decomp decompose(int ni_c, int ntile, int nnbmax, int m_flag, int group) {
+    Decomp d;
+    const int n_iw = cdiv(ni_c, 64);
+    const double target = resident_waves(m_flag, group) * pick_rounds();
+    int ns = (int)std::lround(target / n_iw);
+    ns = std::max(ns, 1);
+    ns = std::min(ns, std::max(ntile, 1));
+    ns = std::max(ns, cdiv(ntile, 1024));
+    d.nslice = ns;
+    const int max_tiles = cdiv(ntile, ns);
+    d.cap = std::max(1, std::min(max_tiles * kTile, nnbmax + 1));
+    return d;
+}
+
+constexpr int kChunk = 32768;  // i-particles per pass-1/pass-2 launch pair
+
+// Next free timing event (created on first use, reused after a read).
+int timing_event() {
+    if (S.ev_used == (int)S.ev_pool.size()) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        S.ev_pool.push_back(e);
+    }
+    return S.ev_used++;
+}
+
+void ensure_regf_scratch(int ni_c, const Decomp &d) {
+    const size_t pairs = (size_t)round_up(ni_c, 64) * d.nslice;
+    S.part.ensure(pairs * 8);
+    S.cnt.ensure(pairs);
+    S.seg.ensure(pairs * d.cap);
+}
+
+// Enqueue the regular-force kernels for device-resident i data.
+void enqueue_regf(int ni, const double *xi, const double *vi, const double *h2,
+                  const double *dtr, double *acc, double *jrk, double *pot,
+                  int nnbmax, int m_flag, int *nnb, int *nnid, int *list,
+                  int *off, int *total, long long list_cap) {
+    const int group = pick_group();
+    const int ntile = cdiv(S.nj, kTile);
+    for (int c0 = 0; c0 < ni; c0 += kChunk) {
+        const int nc = std::min(kChunk, ni - c0);
+        Decomp d = decompose(nc, ntile, nnbmax, m_flag, group);
+        ensure_regf_scratch(nc, d);
+        RegfLaunch L;
+        L.jp = S.jp.p;
+        L.nj = S.nj;
+        L.ntile = ntile;
+        L.jst = S.jst.p;
+        L.xi = xi + 3 * (size_t)c0;
+        L.vi = vi + 3 * (size_t)c0;
+        L.h2 = h2 + c0;
+        L.dtr = dtr + c0;
+        L.ni = nc;
+        L.nslice = d.nslice;
+        L.cap = d.cap;
+        L.part = S.part.p;
+        L.cnt = S.cnt.p;
+        L.seg = S.seg.p;
+        L.acc = acc + 3 * (size_t)c0;
+        L.jrk = jrk + 3 * (size_t)c0;
+        L.pot = pot + c0;
+        L.nnb = nnb + c0;
+        L.nnid = nnid ? nnid + c0 : nullptr;
+        L.list = list;
+        L.off = off + c0;
+        L.total = total;
+        L.list_cap = list_cap;
+        L.nnbmax = nnbmax;
+        L.m_flag = m_flag;
+        L.group = group;
+        L.err = S.err.p;
+        if (ntile == 0) {
+            // No j-particles: one empty slice, zero forces, empty lists.
+            L.nslice = 1;
+            L.cap = 1;
+        }
+        int e0 = -1, e1 = -1;
+        if (S.timing) {
+            e0 = timing_event();
+            HIPCHK(hipEventRecord(S.ev_pool[e0], S.st));
+        }
+        launch_regf_pass1(S.st, L);
+        if (S.timing) {
+            e1 = timing_event();
+            HIPCHK(hipEventRecord(S.ev_pool[e1], S.st));
+            S.ev_pairs.emplace_back(e0, e1);
+            S.ev_pairs_work.push_back((long long)nc * S.nj);
+        }
+        launch_regf_pass2(S.st, L);
+        HIPCHK(hipGetLastError());
+    }
+}
+
+void require_open(const char *what) {
+    if (!S.open) throw Error(std::string(what) + ": library is not open");
+}
+
+void wait_event(hipEvent_t ev, bool &live) {
+    if (live) {
+        HIPCHK(hipEventSynchronize(ev));
+        live = false;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Entry point bodies
+// ---------------------------------------------------------------------------
+
+void do_open(int nbmax, int irank) {
+    S.time_send = S.time_grav = S.time_nb = S.time_out = 0.0;
+    S.numInter = 0;
+    S.icall = S.ini = S.isend = 0;
+    devinit(irank);
+    if (S.open) {
+        fprintf(stderr, "gpunb: it is already open\n");
+        return;
+    }
+    REQUIRE(nbmax >= 0, "nbmax must be >= 0");
+    set_device();
+    const size_t cap = (size_t)round_up(std::max(nbmax, 1), kTile) + kTile;
+    S.jp.ensure(cap);
+    S.x0.ensure(3 * (size_t)std::max(nbmax, 1));
+    S.v0.ensure(3 * (size_t)std::max(nbmax, 1));
+    S.f.ensure(3 * (size_t)std::max(nbmax, 1));
+    S.fd.ensure(3 * (size_t)std::max(nbmax, 1));
+    S.t0.ensure((size_t)std::max(nbmax, 1));
+    S.mass.ensure((size_t)std::max(nbmax, 1));
+    HIPCHK(hipMemsetAsync(S.x0.p, 0, S.x0.n * sizeof(double), S.st));
+    HIPCHK(hipMemsetAsync(S.v0.p, 0, S.v0.n * sizeof(double), S.st));
+    HIPCHK(hipMemsetAsync(S.f.p, 0, S.f.n * sizeof(double), S.st));
+    HIPCHK(hipMemsetAsync(S.fd.p, 0, S.fd.n * sizeof(double), S.st));
+    HIPCHK(hipMemsetAsync(S.t0.p, 0, S.t0.n * sizeof(double), S.st));
+    HIPCHK(hipMemsetAsync(S.mass.p, 0, S.mass.n * sizeof(double), S.st));
+    // Park every record (mass 0, far away) so spare records are harmless.
+    launch_stats_reset(S.st, S.jst.p, nullptr);
+    launch_pack_j(S.st, nullptr, nullptr, nullptr, 0, (int)cap, S.jp.p,
+                  S.jst.p, S.err.p);
+    HIPCHK(hipStreamSynchronize(S.st));
+    S.nbmax = nbmax;
+    S.nj = 0;
+    S.njp = 0;
+    S.open = true;
+    if (!quiet())
+        fprintf(stderr, "# Open MI355X regular force - rank: %d; nbmax: %d\n",
+                irank, nbmax);
+}
+
+void do_close() {
+    if (!S.open) {
+        fprintf(stderr, "gpunb: it is already close\n");
+        return;
+    }
+    set_device();
+    HIPCHK(hipStreamSynchronize(S.st));
+    S.ev_send_live = S.ev_jp_live = false;
+    S.jp.release();
+    S.d_send.release();
+    S.h_send.release();
+    S.x0.release();
+    S.v0.release();
+    S.f.release();
+    S.fd.release();
+    S.t0.release();
+    S.mass.release();
+    S.h_jp.release();
+    S.d_jp.release();
+    S.h_addr.release();
+    S.d_addr.release();
+    S.h_in.release();
+    S.d_in.release();
+    S.part.release();
+    S.cnt.release();
+    S.seg.release();
+    S.d_out.release();
+    S.h_out.release();
+    S.d_iout.release();
+    S.h_iout.release();
+    S.d_list.release();
+    S.h_list.release();
+    S.nbmax = 0;
+    S.nj = 0;
+    S.njp = 0;
+    S.open = false;
+}
+
+void do_send(int nj, const double *m, const double (*x)[3],
+             const double (*v)[3]) {
+    require_open("gpunb_send_");
+    REQUIRE(nj >= 0, "gpunb_send_: nj must be >= 0");
+    REQUIRE(nj <= S.nbmax, "gpunb_send_: nj exceeds nbmax of gpunb_open_");
+    double t0 = wtime();
+    set_device();
+    S.isend++;
+    wait_event(S.ev_send, S.ev_send_live);
+    const size_t n7 = 7 * (size_t)nj;
+    S.h_send.ensure(std::max<size_t>(n7, 7));
+    S.d_send.ensure(std::max<size_t>(n7, 7));
+    if (nj > 0) {
+        memcpy(S.h_send.p, m, nj * sizeof(double));
+        memcpy(S.h_send.p + nj, x, 3 * nj * sizeof(double));
+        memcpy(S.h_send.p + 4 * (size_t)nj, v, 3 * nj * sizeof(double));
+        HIPCHK(hipMemcpyAsync(S.d_send.p, S.h_send.p, n7 * sizeof(double),
+                              hipMemcpyHostToDevice, S.st));
+        HIPCHK(hipEventRecord(S.ev_send, S.st));
+        S.ev_send_live = true;
+    }
+    const int old_pad = round_up(S.nj, kTile);
+    const int pad = std::max(round_up(nj, kTile), old_pad);
+    launch_stats_reset(S.st, S.jst.p, nullptr);
+    launch_pack_j(S.st, S.d_send.p, S.d_send.p + nj, S.d_send.p + 4 * (size_t)nj,
+                  nj, pad, S.jp.p, S.jst.p, S.err.p);
+    HIPCHK(hipGetLastError());
+    S.nj = nj;
+    S.time_send += wtime() - t0;
+}
+
+void do_regf(int ni, const double *h2, const double *dtr, const double (*xi)[3],
+             const double (*vi)[3], double (*acc)[3], double (*jrk)[3],
+             double *pot, int lmax, int nnbmax, int *list, int m_flag,
+             int *nnid) {
+    require_open("gpunb_regf_");
+    REQUIRE(ni >= 0, "gpunb_regf_: ni must be >= 0");
+    REQUIRE(nnbmax >= 0, "gpunb_regf_: nnbmax must be >= 0");
+    REQUIRE(lmax > nnbmax, "gpunb_regf_: lmax must exceed nnbmax");
+    if (ni == 0) return;
+    double t0 = wtime();
+    set_device();
+    S.numInter += (long long)ni * S.nj;
+    S.ini += ni;
+    S.icall++;
+
+    const size_t n8 = 8 * (size_t)ni;
+    S.h_in.ensure(n8);
+    S.d_in.ensure(n8);
+    memcpy(S.h_in.p, xi, 3 * (size_t)ni * sizeof(double));
+    memcpy(S.h_in.p + 3 * (size_t)ni, vi, 3 * (size_t)ni * sizeof(double));
+    memcpy(S.h_in.p + 6 * (size_t)ni, h2, (size_t)ni * sizeof(double));
+    memcpy(S.h_in.p + 7 * (size_t)ni, dtr, (size_t)ni * sizeof(double));
+    HIPCHK(hipMemcpyAsync(S.d_in.p, S.h_in.p, n8 * sizeof(double),
+                          hipMemcpyHostToDevice, S.st));
+
+    const long long list_cap = std::max<long long>(1, (long long)ni * nnbmax);
+    S.d_out.ensure(7 * (size_t)ni);
+    S.h_out.ensure(7 * (size_t)ni);
+    S.d_iout.ensure(3 * (size_t)ni + 1);
+    S.h_iout.ensure(3 * (size_t)ni + 1);
+    S.d_list.ensure((size_t)list_cap);
+    double *d_acc = S.d_out.p, *d_jrk = d_acc + 3 * (size_t)ni,
+           *d_pot = d_acc + 6 * (size_t)ni;
+    int *d_nnb = S.d_iout.p, *d_off = d_nnb + ni, *d_nnid = d_off + ni,
+        *d_total = d_nnid + ni;
+    launch_zero_int(S.st, d_total, 1);
+    const double *dx = S.d_in.p, *dv = dx + 3 * (size_t)ni,
+                 *dh = dx + 6 * (size_t)ni, *dt = dx + 7 * (size_t)ni;
+    enqueue_regf(ni, dx, dv, dh, dt, d_acc, d_jrk, d_pot, nnbmax, m_flag,
+                 d_nnb, d_nnid, S.d_list.p, d_off, d_total, list_cap);
+
+    // Results back: fixed-size outputs plus a speculative prefix of the
+    // compact list (sized from the previous call); the rest, if any, after
+    // the synchronisation.
+    long long est = std::max<long long>((long long)ni * 64, S.list_est);
+    est = std::min(est, list_cap);
+    S.h_list.ensure((size_t)est);
+    HIPCHK(hipMemcpyAsync(S.h_out.p, S.d_out.p, 7 * (size_t)ni * sizeof(double),
+                          hipMemcpyDeviceToHost, S.st));
+    HIPCHK(hipMemcpyAsync(S.h_iout.p, S.d_iout.p,
+                          (3 * (size_t)ni + 1) * sizeof(int),
+                          hipMemcpyDeviceToHost, S.st));
+    HIPCHK(hipMemcpyAsync(S.h_list.p, S.d_list.p, (size_t)est * sizeof(int),
+                          hipMemcpyDeviceToHost, S.st));
+    err_fetch();
+    HIPCHK(hipStreamSynchronize(S.st));
+    err_check();
+    double t1 = wtime();
+    S.time_grav += t1 - t0;
+
+    const int *h_nnb = S.h_iout.p, *h_off = h_nnb + ni, *h_nnid = h_off + ni;
+    const long long total = S.h_iout.p[3 * (size_t)ni];
+    S.list_est = total + total / 4 + 64;
+    if (total > est) {
+        // Grow the pinned buffer, keeping the prefix already copied.
+        HostBuf<int> nb;
+        nb.ensure((size_t)total);
+        memcpy(nb.p, S.h_list.p, (size_t)est * sizeof(int));
+        S.h_list.release();
+        S.h_list = nb;
+        nb.p = nullptr;
+        HIPCHK(hipMemcpyAsync(S.h_list.p + est, S.d_list.p + est,
+                              (size_t)(total - est) * sizeof(int),
+                              hipMemcpyDeviceToHost, S.st));
+        HIPCHK(hipStreamSynchronize(S.st));
+    }
+    double t2 = wtime();
+    S.time_nb += t2 - t1;
+
+    const double *h_acc = S.h_out.p, *h_jrk = h_acc + 3 * (size_t)ni,
+                 *h_pot = h_acc + 6 * (size_t)ni;
+    memcpy(acc, h_acc, 3 * (size_t)ni * sizeof(double));
+    memcpy(jrk, h_jrk, 3 * (size_t)ni * sizeof(double));
+    memcpy(pot, h_pot, (size_t)ni * sizeof(double));
+    for (int i = 0; i < ni; ++i) {
+        int *row = list + (size_t)i * lmax;
+        const int nnb = h_nnb[i];
+        row[0] = nnb;
+        if (nnb > 0)
+            memcpy(row + 1, S.h_list.p + h_off[i], (size_t)nnb * sizeof(int));
+    }
+    if (nnid) memcpy(nnid, h_nnid, (size_t)ni * sizeof(int));
+    S.time_out += wtime() - t2;
+}
+
+void do_profile(int irank) {
+    if (S.icall && !quiet()) {
+        fprintf(stderr,
+                "[R.%d-D.%d MI355X Reg.F ] Nsend %d  Ngrav %d  <Ni> %d   "
+                "send(s) %f grav(s) %f  nb(s) %f  out(s) %f  Perf.(Gflops) %f\n",
+                irank, S.dev, S.isend, S.icall, S.isend ? S.ini / S.isend : S.ini,
+                S.time_send, S.time_grav, S.time_nb, S.time_out,
+                S.time_grav > 0 ? 60.e-9 * S.numInter / S.time_grav : 0.0);
+    }
+    S.time_send = S.time_grav = S.time_nb = S.time_out = 0.0;
+    S.numInter = 0;
+    S.icall = S.ini = S.isend = 0;
+}
+
+void do_pot(int irank, int istart, int ni, int n, const double *m,
+            const double (*x)[3], double *pot) {
+    devinit(irank);
+    REQUIRE(n >= 0 && ni >= 0, "gpupot_: n and ni must be >= 0");
+    REQUIRE(istart >= 1 || ni == 0, "gpupot_: istart is 1-based");
+    REQUIRE((long long)istart - 1 + ni <= n, "gpupot_: istart-1+ni exceeds n");
+    if (ni == 0) return;
+    double t0 = wtime();
+    set_device();
+    const int ntile = cdiv(n, kTile);
+    const int n_pad = ntile * kTile;
+    S.h_pin.ensure(4 * (size_t)n);
+    S.d_pin.ensure(4 * (size_t)n);
+    S.prec.ensure((size_t)n_pad + kTile);
+    memcpy(S.h_pin.p, m, (size_t)n * sizeof(double));
+    memcpy(S.h_pin.p + n, x, 3 * (size_t)n * sizeof(double));
+    HIPCHK(hipMemcpyAsync(S.d_pin.p, S.h_pin.p, 4 * (size_t)n * sizeof(double),
+                          hipMemcpyHostToDevice, S.st));
+    launch_pack_p(S.st, S.d_pin.p, S.d_pin.p + n, n, n_pad + kTile, S.prec.p,
+                  S.err.p);
+    if (!S.pot_resident) S.pot_resident = pot_resident_waves();
+    const int target = std::max(S.pot_resident, 1024) * 2;
+    const int n_iw = cdiv(ni, 64);
+    int ns = std::max(1, (int)std::lround((double)target / n_iw));
+    ns = std::min(ns, std::max(ntile, 1));
+    S.d_ppart.ensure((size_t)round_up(ni, 64) * ns);
+    S.d_pout.ensure((size_t)ni);
+    S.h_pout.ensure((size_t)ni);
+    launch_pot(S.st, S.prec.p, n, ntile, istart - 1, ni, ns, S.d_ppart.p,
+               S.d_pout.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(S.h_pout.p, S.d_pout.p, (size_t)ni * sizeof(double),
+                          hipMemcpyDeviceToHost, S.st));
+    err_fetch();
+    HIPCHK(hipStreamSynchronize(S.st));
+    err_check();
+    memcpy(pot, S.h_pout.p, (size_t)ni * sizeof(double));
+    if (!quiet())
+        fprintf(stderr, "[R.%d MI355X Pot.A] Ni %d  NTOT %d  pot(s) %f\n",
+                irank, ni, n, wtime() - t0);
+}
+
+void do_jpset(int n, const int *addr, const double (*x0)[3],
+              const double (*v0)[3], const double (*f)[3],
+              const double (*fd)[3], const double *mass, const double *t0) {
+    require_open("gpunb_jpset_");
+    REQUIRE(n >= 0, "gpunb_jpset_: n must be >= 0");
+    if (n == 0) return;
+    int amax = 0;
+    for (int k = 0; k < n; ++k) {
+        REQUIRE(addr[k] >= 1 && addr[k] <= S.nbmax,
+                "gpunb_jpset_: address outside 1..nbmax");
+        amax = std::max(amax, addr[k]);
+    }
+    set_device();
+    wait_event(S.ev_jp, S.ev_jp_live);
+    S.h_jp.ensure(14 * (size_t)n);
+    S.d_jp.ensure(14 * (size_t)n);
+    S.h_addr.ensure((size_t)n);
+    S.d_addr.ensure((size_t)n);
+    for (int k = 0; k < n; ++k) {
+        double *r = S.h_jp.p + 14 * (size_t)k;
+        for (int c = 0; c < 3; ++c) {
+            r[c] = x0[k][c];
+            r[3 + c] = v0[k][c];
+            r[6 + c] = f[k][c];
+            r[9 + c] = fd[k][c];
+        }
+        r[12] = mass[k];
+        r[13] = t0[k];
+    }
+    memcpy(S.h_addr.p, addr, (size_t)n * sizeof(int));
+    HIPCHK(hipMemcpyAsync(S.d_jp.p, S.h_jp.p, 14 * (size_t)n * sizeof(double),
+                          hipMemcpyHostToDevice, S.st));
+    HIPCHK(hipMemcpyAsync(S.d_addr.p, S.h_addr.p, (size_t)n * sizeof(int),
+                          hipMemcpyHostToDevice, S.st));
+    HIPCHK(hipEventRecord(S.ev_jp, S.st));
+    S.ev_jp_live = true;
+    launch_jpset(S.st, n, S.d_addr.p, S.d_jp.p, S.x0.p, S.v0.p, S.f.p, S.fd.p,
+                 S.mass.p, S.t0.p, S.err.p);
+    HIPCHK(hipGetLastError());
+    S.njp = std::max(S.njp, amax);
+}
+
+void do_predict(double t) {
+    require_open("gpunb_predict_");
+    set_device();
+    const int pad = std::max(round_up(S.njp, kTile), round_up(S.nj, kTile));
+    launch_stats_reset(S.st, S.jst.p, nullptr);
+    launch_predict(S.st, t, S.njp, pad, S.x0.p, S.v0.p, S.f.p, S.fd.p, S.t0.p,
+                   S.mass.p, S.jp.p, S.jst.p);
+    HIPCHK(hipGetLastError());
+    S.nj = S.njp;
+}
+
+void do_jpget(int n, double (*x)[3], double (*v)[3]) {
+    require_open("gpunb_jpget_");
+    REQUIRE(n >= 0 && n <= S.nj, "gpunb_jpget_: n outside 0..nj");
+    if (n == 0) return;
+    set_device();
+    S.d_pout.ensure(6 * (size_t)n);
+    S.h_pout.ensure(6 * (size_t)n);
+    launch_jget(S.st, n, S.jp.p, S.d_pout.p, S.d_pout.p + 3 * (size_t)n);
+    HIPCHK(hipMemcpyAsync(S.h_pout.p, S.d_pout.p, 6 * (size_t)n * sizeof(double),
+                          hipMemcpyDeviceToHost, S.st));
+    HIPCHK(hipStreamSynchronize(S.st));
+    memcpy(x, S.h_pout.p, 3 * (size_t)n * sizeof(double));
+    memcpy(v, S.h_pout.p + 3 * (size_t)n, 3 * (size_t)n * sizeof(double));
+}
+
+}  // namespace
+}  // namespace gpunb
+
+using namespace gpunb;
+
+// Exported symbols (the library is built with -fvisibility=hidden).
+#define GPUNB_EXPORT __attribute__((visibility("default")))
+
+extern "C" {
+
+GPUNB_EXPORT void gpunb_devinit_(int *irank) {
+    guarded("gpunb_devinit_", [&] { devinit(*irank); });
+}
+
+GPUNB_EXPORT void gpunb_open_(int *nbmax, int *irank) {
+    guarded("gpunb_open_", [&] { do_open(*nbmax, *irank); });
+}
+
+GPUNB_EXPORT void gpunb_close_(void) {
+    guarded("gpunb_close_", [&] { do_close(); });
+}
+
+GPUNB_EXPORT void gpunb_send_(int *nj, double *mj, double (*xj)[3], double (*vj)[3]) {
+    guarded("gpunb_send_", [&] { do_send(*nj, mj, xj, vj); });
+}
+
+GPUNB_EXPORT void gpunb_regf_(int *ni, double *h2, double *dtr, double (*xi)[3],
+                 double (*vi)[3], double (*acc)[3], double (*jrk)[3],
+                 double *pot, int *lmax, int *nnbmax, int *list, int *m_flag) {
+    guarded("gpunb_regf_", [&] {
+        do_regf(*ni, h2, dtr, xi, vi, acc, jrk, pot, *lmax, *nnbmax, list,
+                *m_flag, nullptr);
+    });
+}
+
+GPUNB_EXPORT void gpunb_regfnn_(int *ni, double *h2, double *dtr, double (*xi)[3],
+                   double (*vi)[3], double (*acc)[3], double (*jrk)[3],
+                   double *pot, int *lmax, int *nnbmax, int *list,
+                   int *m_flag, int *nnid) {
+    guarded("gpunb_regfnn_", [&] {
+        do_regf(*ni, h2, dtr, xi, vi, acc, jrk, pot, *lmax, *nnbmax, list,
+                *m_flag, nnid);
+    });
+}
+
+GPUNB_EXPORT void gpunb_profile_(int *irank) {
+    guarded("gpunb_profile_", [&] { do_profile(*irank); });
+}
+
+GPUNB_EXPORT void gpupot_(int *irank, int *istart, int *ni, int *n, double *m,
+             double (*x)[3], double *pot) {
+    guarded("gpupot_", [&] { do_pot(*irank, *istart, *ni, *n, m, x, pot); });
+}
+
+GPUNB_EXPORT void gpunb_jpset_(int *n, int *addr, double (*x0)[3], double (*v0)[3],
+                  double (*f)[3], double (*fdot)[3], double *mass,
+                  double *t0) {
+    guarded("gpunb_jpset_",
+            [&] { do_jpset(*n, addr, x0, v0, f, fdot, mass, t0); });
+}
+
+GPUNB_EXPORT void gpunb_predict_(double *time) {
+    guarded("gpunb_predict_", [&] { do_predict(*time); });
+}
+
+GPUNB_EXPORT void gpunb_jpget_(int *n, double (*x)[3], double (*v)[3]) {
+    guarded("gpunb_jpget_", [&] { do_jpget(*n, x, v); });
+}
+
+GPUNB_EXPORT void gpunb_set_abort_(int *on) {
+    g_abort = (*on != 0);
+    g_abort_set = true;
+}
+
+GPUNB_EXPORT int gpunb_last_error(char *buf, int len) {
+    const int n = (int)g_last_error.size();
+    if (buf && len > 0) {
+        const int k = std::min(n, len - 1);
+        memcpy(buf, g_last_error.data(), (size_t)k);
+        buf[k] = '\0';
+    }
+    g_last_error.clear();
+    return n;
+}
+
+GPUNB_EXPORT int gpunb_regf_dev(int ni, const double *d_h2, const double *d_dtr,
+                   const double *d_xi, const double *d_vi, double *d_acc,
+                   double *d_jrk, double *d_pot, int nnbmax, int m_flag,
+                   int *d_nnb, int *d_nnid, int *d_list, int *d_off,
+                   int *d_total, long long list_cap) {
+    return guarded("gpunb_regf_dev", [&] {
+        require_open("gpunb_regf_dev");
+        REQUIRE(ni >= 0 && nnbmax >= 0, "gpunb_regf_dev: bad sizes");
+        set_device();
+        launch_zero_int(S.st, d_total, 1);
+        if (ni == 0) return;
+        S.numInter += (long long)ni * S.nj;
+        S.ini += ni;
+        S.icall++;
+        enqueue_regf(ni, d_xi, d_vi, d_h2, d_dtr, d_acc, d_jrk, d_pot, nnbmax,
+                     m_flag, d_nnb, d_nnid, d_list, d_off, d_total, list_cap);
+    });
+}
+
+GPUNB_EXPORT int gpunb_sync(void) {
+    return guarded("gpunb_sync", [&] {
+        if (!S.devinit) return;
+        set_device();
+        err_fetch();
+        HIPCHK(hipStreamSynchronize(S.st));
+        err_check();
+    });
+}
+
+GPUNB_EXPORT void *gpunb_stream(void) { return (void *)S.st; }
+
+GPUNB_EXPORT const char *gpunb_version(void) {
+    return "gpunb-mi355x 0.1 (gfx950, FP64-difference regular force)";
+}
+
+GPUNB_EXPORT int gpunb_timing(int enable) {
+    return guarded("gpunb_timing", [&] {
+        if (S.devinit) {
+            set_device();
+            HIPCHK(hipStreamSynchronize(S.st));
+        }
+        S.timing = enable != 0;
+        S.ev_pairs.clear();
+        S.ev_pairs_work.clear();
+        S.ev_used = 0;
+    });
+}
+
+GPUNB_EXPORT int gpunb_timing_read(double *ms_total, long long *launches,
+                                   long long *pairs) {
+    return guarded("gpunb_timing_read", [&] {
+        double t = 0.0;
+        long long w = 0;
+        if (S.devinit && !S.ev_pairs.empty()) {
+            set_device();
+            HIPCHK(hipStreamSynchronize(S.st));
+            for (size_t k = 0; k < S.ev_pairs.size(); ++k) {
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, S.ev_pool[S.ev_pairs[k].first],
+                                           S.ev_pool[S.ev_pairs[k].second]));
+                t += ms;
+                w += S.ev_pairs_work[k];
+            }
+        }
+        if (ms_total) *ms_total = t;
+        if (launches) *launches = (long long)S.ev_pairs.size();
+        if (pairs) *pairs = w;
+        S.ev_pairs.clear();
+        S.ev_pairs_work.clear();
+        S.ev_used = 0;
+    });
+}
+
+GPUNB_EXPORT int gpunb_set_tuning(int group, double rounds) {
+    return guarded("gpunb_set_tuning", [&] {
+        REQUIRE(group == 0 || group == 2 || group == 4,
+                "gpunb_set_tuning: group must be 0, 2 or 4");
+        REQUIRE(rounds >= 0.0, "gpunb_set_tuning: rounds must be >= 0");
+        S.group = group;
+        S.rounds = rounds;
+    });
+}
+
+}  // extern "C"

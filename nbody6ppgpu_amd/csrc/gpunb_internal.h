@@ -1,0 +1,105 @@
+// gpunb_internal.h -- shared declarations between the HIP kernels
+// (kernels.hip) and the host side of the C-ABI (gpunb.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace gpunb {
+
+// One j-particle in HBM: 64 bytes, so a wavefront fetches a whole record with
+// one scalar load (s_load_dwordx16) and feeds it to VALU ops as SGPR operands.
+// FP64 position, velocity and mass are kept because the pair differences and
+// the neighbour predicate are evaluated from FP64 values; m32 is the FP32 mass
+// used by the force sums.
+struct alignas(64) JRec {
+    double x, y, z;
+    double vx, vy, vz;
+    double m;
+    float m32;
+    float pad;
+};
+static_assert(sizeof(JRec) == 64, "JRec must be 64 bytes");
+
+// One particle for the potential kernel: 32 bytes (s_load_dwordx8).
+struct alignas(32) PRec {
+    double x, y, z;
+    float m32;
+    float pad;
+};
+static_assert(sizeof(PRec) == 32, "PRec must be 32 bytes");
+
+// Statistics of the current j set, computed on the device at send/predict:
+// max |v_j| and min positive m_j, stored as the bit patterns of the FP64
+// values (non-negative doubles order like their bit patterns, so integer
+// atomicMax/atomicMin reduce them).
+struct JStats {
+    unsigned long long vmax_bits;
+    unsigned long long mmin_bits;
+};
+
+// Device error bits (OR-ed by kernels, checked by the host at sync points).
+enum : unsigned {
+    ERR_NAN_J = 1u,      // NaN in j data (send / jpset)
+    ERR_NAN_I = 2u,      // NaN in i data (regf)
+    ERR_LIST_CAP = 4u,   // compact list buffer too small (device entry point)
+};
+
+// Far-away parking place of the padding j records (mass 0).
+constexpr double kPadPos = 1.0e18;
+
+// Tile of j-particles between FP32 -> FP64 folds of the partial sums.
+constexpr int kTile = 64;
+
+struct RegfLaunch {
+    // j side
+    const JRec *jp;
+    int nj;          // real j count
+    int ntile;       // padded j count / kTile
+    const JStats *jst;
+    // i side (device, AoS like the ABI)
+    const double *xi, *vi, *h2, *dtr;
+    int ni;
+    // decomposition
+    int nslice;      // j slices per i-wave
+    int cap;         // neighbour segment capacity per (i, slice)
+    // pass-1 scratch
+    double *part;    // [ni_pad][nslice][8]
+    int *cnt;        // [ni_pad][nslice]
+    uint16_t *seg;   // [ni_pad][nslice][cap]
+    // outputs
+    double *acc, *jrk, *pot;
+    int *nnb, *nnid, *list, *off, *total;
+    long long list_cap;
+    int nnbmax;
+    int m_flag;
+    int group;       // pairs per branch-free group of pass 1 (2 or 4)
+    unsigned *err;
+};
+
+// Kernel launchers (kernels.hip).  All asynchronous on `st`.
+void launch_pack_j(hipStream_t st, const double *m, const double *x,
+                   const double *v, int nj, int nj_pad, JRec *jp, JStats *jst,
+                   unsigned *err);
+void launch_stats_reset(hipStream_t st, JStats *jst, int *total);
+void launch_predict(hipStream_t st, double t, int nj, int nj_pad,
+                    const double *x0, const double *v0, const double *f,
+                    const double *fd, const double *t0, const double *m,
+                    JRec *jp, JStats *jst);
+void launch_jpset(hipStream_t st, int n, const int *addr, const double *src,
+                  double *x0, double *v0, double *f, double *fd, double *mass,
+                  double *t0, unsigned *err);
+void launch_jget(hipStream_t st, int n, const JRec *jp, double *x, double *v);
+void launch_regf_pass1(hipStream_t st, const RegfLaunch &L);
+void launch_regf_pass2(hipStream_t st, const RegfLaunch &L);
+void launch_zero_int(hipStream_t st, int *p, int n);
+void launch_pot(hipStream_t st, const PRec *p, int n, int ntile, int i0,
+                int ni, int nslice, double *part, double *pot);
+void launch_pack_p(hipStream_t st, const double *m, const double *x, int n,
+                   int n_pad, PRec *p, unsigned *err);
+
+// Occupancy-derived resident-wave capacity of the regular-force kernel.
+int regf_resident_waves(int m_flag, int group);
+int pot_resident_waves();
+
+}  // namespace gpunb

@@ -1,0 +1,715 @@
+// kernels.hip -- CDNA4 (gfx950) kernels of the regular-force engine.
+//
+// Compiled with -ffp-contract=off: every expression is evaluated as written.
+// The FP32 force arithmetic uses explicit fmaf(); the FP64 neighbour
+// predicate and the predictor contain no fused operation, so they reproduce
+// the FP64 restatement of the reference bit for bit.
+//
+// Kernels
+//   k_pack_j       FP64 AoS (m, x, v) -> 64-byte JRec records; NaN check;
+//                  max |v| and min positive m of the j set.
+//   k_predict      Hermite predictor of every j (xbpredall.f:18-26) -> JRec.
+//   k_jpset        scatter of corrected Hermite states into the predictor SoA.
+//   k_regf_pass1   pair sums + neighbour capture over one j slice per wave.
+//   k_regf_pass2   per-i FP64 reduction of the slice partials, ordered list
+//                  compaction and nearest neighbour.
+//   k_pot_pass1/2  O(N^2) potential (gpupot.gpu.cu:32-57).
+#include "gpunb_internal.h"
+
+#include <cfloat>
+#include <climits>
+#include <cmath>
+
+
+
+namespace gpunb {
+
+namespace {
+
+constexpr float kU32 = 5.9604644775390625e-08f;  // 2^-24, FP32 unit roundoff
+
+__device__ inline unsigned long long dbits(double v) {
+    return (unsigned long long)__double_as_longlong(v);
+}
+__device__ inline double bitsd(unsigned long long b) {
+    return __longlong_as_double((long long)b);
+}
+
+// Squared norm with the association ((a*a + b*b) + c*c) and no FMA.
+__device__ inline double sq3d(double a, double b, double c) {
+    double s = a * a;
+    s = s + b * b;
+    s = s + c * c;
+    return s;
+}
+
+__device__ inline double wave_max(double v) {
+    for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ inline double wave_min(double v) {
+    for (int o = 32; o >= 1; o >>= 1) v = fmin(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ inline int wave_sum(int v) {
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+__device__ inline int wave_incl_scan(int v, int lane) {
+    for (int o = 1; o < 64; o <<= 1) {
+        int u = __shfl_up(v, o);
+        if (lane >= o) v += u;
+    }
+    return v;
+}
+
+__device__ inline void set_pad(JRec &r) {
+    r.x = kPadPos;
+    r.y = kPadPos;
+    r.z = kPadPos;
+    r.vx = 0.0;
+    r.vy = 0.0;
+    r.vz = 0.0;
+    r.m = 0.0;
+    r.m32 = 0.0f;
+    r.pad = 0.0f;
+}
+
+// Fold the per-thread speed and mass extrema of one wave into JStats.
+__device__ inline void stats_fold(JStats *jst, double vmag, double mpos) {
+    vmag = wave_max(vmag);
+    mpos = wave_min(mpos);
+    if ((threadIdx.x & 63) == 0) {
+        if (vmag > 0.0) atomicMax(&jst->vmax_bits, dbits(vmag));
+        if (mpos < INFINITY) atomicMin(&jst->mmin_bits, dbits(mpos));
+    }
+}
+
+// |v| rounded up by a few ulps, so the bound built on it is safe.
+__device__ inline double speed_up(double vx, double vy, double vz) {
+    return sqrt(sq3d(vx, vy, vz)) * (1.0 + 1e-15);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// j-set preparation
+// ---------------------------------------------------------------------------
+
+__global__ void k_stats_reset(JStats *jst, int *total) {
+    if (threadIdx.x == 0) {
+        jst->vmax_bits = 0ull;
+        jst->mmin_bits = dbits(INFINITY);
+        if (total) *total = 0;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_pack_j(
+    const double *__restrict__ m, const double *__restrict__ x,
+    const double *__restrict__ v, int nj, int nj_pad, JRec *__restrict__ jp,
+    JStats *__restrict__ jst, unsigned *__restrict__ err) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    double vmag = 0.0, mpos = INFINITY;
+    if (j < nj) {
+        JRec r;
+        r.x = x[3 * (size_t)j + 0];
+        r.y = x[3 * (size_t)j + 1];
+        r.z = x[3 * (size_t)j + 2];
+        r.vx = v[3 * (size_t)j + 0];
+        r.vy = v[3 * (size_t)j + 1];
+        r.vz = v[3 * (size_t)j + 2];
+        r.m = m[j];
+        r.m32 = (float)r.m;
+        r.pad = 0.0f;
+        const double chk = r.x + r.y + r.z + r.vx + r.vy + r.vz + r.m;
+        if (chk != chk) atomicOr(err, ERR_NAN_J);
+        jp[j] = r;
+        vmag = speed_up(r.vx, r.vy, r.vz);
+        if (r.m > 0.0) mpos = r.m;
+    } else if (j < nj_pad) {
+        JRec r;
+        set_pad(r);
+        jp[j] = r;
+    }
+    stats_fold(jst, vmag, mpos);
+}
+
+__global__ __launch_bounds__(256) void k_predict(
+    double t, int nj, int nj_pad, const double *__restrict__ x0,
+    const double *__restrict__ v0, const double *__restrict__ f,
+    const double *__restrict__ fd, const double *__restrict__ t0,
+    const double *__restrict__ m, JRec *__restrict__ jp,
+    JStats *__restrict__ jst) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    double vmag = 0.0, mpos = INFINITY;
+    if (j < nj) {
+        // xbpredall.f:18-26, evaluated exactly as written (no FMA).
+        const double s = t - t0[j];
+        const double s1 = 1.5 * s;
+        const double s2 = 2.0 * s;
+        double xo[3], vo[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const size_t q = 3 * (size_t)j + k;
+            xo[k] = ((fd[q] * s + f[q]) * s + v0[q]) * s + x0[q];
+            vo[k] = (fd[q] * s1 + f[q]) * s2 + v0[q];
+        }
+        JRec r;
+        r.x = xo[0];
+        r.y = xo[1];
+        r.z = xo[2];
+        r.vx = vo[0];
+        r.vy = vo[1];
+        r.vz = vo[2];
+        r.m = m[j];
+        r.m32 = (float)r.m;
+        r.pad = 0.0f;
+        jp[j] = r;
+        vmag = speed_up(r.vx, r.vy, r.vz);
+        if (r.m > 0.0) mpos = r.m;
+    } else if (j < nj_pad) {
+        JRec r;
+        set_pad(r);
+        jp[j] = r;
+    }
+    stats_fold(jst, vmag, mpos);
+}
+
+// src holds n records of 14 doubles: x0[3] v0[3] f[3] fdot[3] m t0.
+__global__ __launch_bounds__(256) void k_jpset(
+    int n, const int *__restrict__ addr, const double *__restrict__ src,
+    double *__restrict__ x0, double *__restrict__ v0, double *__restrict__ f,
+    double *__restrict__ fd, double *__restrict__ mass,
+    double *__restrict__ t0, unsigned *__restrict__ err) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int j = addr[k] - 1;
+    const double *s = src + 14 * (size_t)k;
+    double chk = 0.0;
+#pragma unroll
+    for (int c = 0; c < 14; ++c) chk += s[c];
+    if (chk != chk) atomicOr(err, ERR_NAN_J);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        x0[3 * (size_t)j + c] = s[c];
+        v0[3 * (size_t)j + c] = s[3 + c];
+        f[3 * (size_t)j + c] = s[6 + c];
+        fd[3 * (size_t)j + c] = s[9 + c];
+    }
+    mass[j] = s[12];
+    t0[j] = s[13];
+}
+
+__global__ __launch_bounds__(256) void k_jget(int n, const JRec *__restrict__ jp,
+                                              double *__restrict__ x,
+                                              double *__restrict__ v) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const JRec r = jp[j];
+    x[3 * (size_t)j + 0] = r.x;
+    x[3 * (size_t)j + 1] = r.y;
+    x[3 * (size_t)j + 2] = r.z;
+    v[3 * (size_t)j + 0] = r.vx;
+    v[3 * (size_t)j + 1] = r.vy;
+    v[3 * (size_t)j + 2] = r.vz;
+}
+
+__global__ void k_zero_int(int *p, int n) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Regular force, pass 1: one wavefront = 64 i-particles (one per lane) x one
+// contiguous j slice.  j records are wave-uniform, so they arrive through the
+// scalar cache as SGPR operands: no LDS and no VGPRs are spent on them.
+//
+// Per pair (fast path): FP64 differences dx = x_j - x_i, dv = v_j - v_i
+// (exactly the oracle's), rounded to FP32; FP32 r2, r2p, dx.dv, one v_rsq_f32
+// and the force sums.  The neighbour test first compares the FP32
+// min(r2, r2p) with a per-lane threshold that is provably above the FP64
+// threshold by more than the FP32 evaluation error; only lanes below it take
+// the FP64 path, which evaluates the oracle's predicate bit for bit.  Pairs
+// that are no neighbours therefore never pay for FP64 arithmetic beyond the
+// six differences, and the lists are exactly the FP64 lists.
+//
+// The FP32 partial sums are folded into FP64 accumulators every kTile pairs.
+// ---------------------------------------------------------------------------
+
+template <bool MFLAG, int G>
+__global__ __launch_bounds__(64) void k_regf_pass1(
+    const JRec *__restrict__ jp, int nj, int ntile,
+    const JStats *__restrict__ jst, const double *__restrict__ xi,
+    const double *__restrict__ vi, const double *__restrict__ h2,
+    const double *__restrict__ dtr, int ni, int n_iw, int nslice, int cap,
+    double *__restrict__ part, int *__restrict__ cnt,
+    uint16_t *__restrict__ seg, unsigned *__restrict__ err) {
+    // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
+    // contiguous range of tasks so that it streams only its share of the j
+    // array through its own L2.  Tasks are slice-major.
+    const int ntask = n_iw * nslice;
+    const int per = (ntask + 7) >> 3;
+    const int task = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    if (task >= ntask) return;
+    const int s = task / n_iw;
+    const int iw = task - s * n_iw;
+    const int lane = threadIdx.x;
+    const int i = iw * 64 + lane;
+    const bool act = i < ni;
+
+    const int tb = (int)(((long long)ntile * s) / nslice);
+    const int te = (int)(((long long)ntile * (s + 1)) / nslice);
+    const int jbeg = tb * kTile;
+
+    double x0 = 0.0, y0 = 0.0, z0 = 0.0, u0 = 0.0, w0 = 0.0, q0 = 0.0;
+    double h2d = 0.0, dtrd = 0.0;
+    float dtr32 = 0.0f;
+    float thr = -1.0f;  // inactive lanes never enter the FP64 path
+    if (act) {
+        x0 = xi[3 * (size_t)i + 0];
+        y0 = xi[3 * (size_t)i + 1];
+        z0 = xi[3 * (size_t)i + 2];
+        u0 = vi[3 * (size_t)i + 0];
+        w0 = vi[3 * (size_t)i + 1];
+        q0 = vi[3 * (size_t)i + 2];
+        h2d = h2[i];
+        dtrd = dtr[i];
+        const double chk = x0 + y0 + z0 + u0 + w0 + q0 + h2d + dtrd;
+        if (chk != chk) atomicOr(err, ERR_NAN_I);
+        dtr32 = (float)dtrd;
+        // FP32 screening threshold.  With rho = |dtr| (|v_i| + max|v_j|) /
+        // sqrt(H), H the FP64 threshold (h2, or m_j h2 >= m_min h2), the FP32
+        // values of r2 and r2p exceed their FP64 values by at most a relative
+        // 5.1u and by (1 + 4u(1 + 2 rho))^2 (1 + 3u) - 1 respectively whenever
+        // the FP64 value is below H.  The threshold doubles that margin.
+        if (h2d > 0.0) {
+            const double vmax = bitsd(jst->vmax_bits);
+            double hmin = h2d;
+            if (MFLAG) {
+                const double mmin = bitsd(jst->mmin_bits);
+                hmin = (mmin < INFINITY) ? h2d * mmin : h2d;
+            }
+            const double u = (double)kU32;
+            const double rho =
+                fabs(dtrd) * (speed_up(u0, w0, q0) + vmax) / sqrt(hmin);
+            const double g = 1.0 + 4.0 * u * (1.0 + 2.0 * rho);
+            const double eps = 2.0 * (g * g * (1.0 + 3.0 * u) - 1.0) +
+                               (MFLAG ? 24.0 : 16.0) * u;
+            const double tt = h2d * (1.0 + eps);
+            float t32 = (float)tt;
+            if ((double)t32 < tt) t32 = nextafterf(t32, INFINITY);
+            thr = fmaxf(t32, FLT_MIN);
+        } else {
+            thr = FLT_MIN;  // only r2 == 0 is screened (never a neighbour)
+        }
+    }
+
+    float ax = 0.f, ay = 0.f, az = 0.f, jx = 0.f, jy = 0.f, jz = 0.f, ph = 0.f;
+    double Ax = 0.0, Ay = 0.0, Az = 0.0, Jx = 0.0, Jy = 0.0, Jz = 0.0, Ph = 0.0;
+    int c = 0;
+    uint16_t *sg = seg + ((size_t)(act ? i : 0) * nslice + s) * cap;
+
+    // FP32 force terms of one pair, shared by the screened fast path and the
+    // FP64 resolution of band pairs.
+    auto pair_terms = [&](const JRec &J, float fx, float fy, float fz,
+                          float gx, float gy, float gz, float r2, float rinv) {
+        const float rv = fmaf(fz, gz, fmaf(fy, gy, fx * gx));
+        const float rinv2 = rinv * rinv;
+        const float mr = J.m32 * rinv;
+        const float mr3 = mr * rinv2;
+        const float al = (-3.0f * rv) * rinv2;
+        (void)r2;
+        ph += mr;
+        ax = fmaf(mr3, fx, ax);
+        ay = fmaf(mr3, fy, ay);
+        az = fmaf(mr3, fz, az);
+        jx = fmaf(mr3, fmaf(al, fx, gx), jx);
+        jy = fmaf(mr3, fmaf(al, fy, gy), jy);
+        jz = fmaf(mr3, fmaf(al, fz, gz), jz);
+    };
+
+    // Resolution of one screened pair in FP64: the oracle's predicate bit for
+    // bit; a neighbour is appended to the lane's segment, anything else adds
+    // the FP32 force term the screened fast path left out.
+    auto resolve = [&](const JRec &J, int j) {
+        const double dx = J.x - x0, dy = J.y - y0, dz = J.z - z0;
+        const double dvx = J.vx - u0, dvy = J.vy - w0, dvz = J.vz - q0;
+        const double r2d = sq3d(dx, dy, dz);
+        const double pxd = dx + dtrd * dvx;
+        const double pyd = dy + dtrd * dvy;
+        const double pzd = dz + dtrd * dvz;
+        const double r2pd = sq3d(pxd, pyd, pzd);
+        const double mnd = r2d < r2pd ? r2d : r2pd;
+        const double H = MFLAG ? J.m * h2d : h2d;
+        if ((mnd < H) && (j < nj)) {
+            if (c < cap) sg[c] = (uint16_t)(j - jbeg);
+            ++c;
+        } else {
+            const float fx = (float)dx, fy = (float)dy, fz = (float)dz;
+            const float r2 = fmaf(fz, fz, fmaf(fy, fy, fx * fx));
+            if (r2 != 0.0f)
+                pair_terms(J, fx, fy, fz, (float)dvx, (float)dvy, (float)dvz,
+                           r2, __builtin_amdgcn_rsqf(r2));
+        }
+    };
+
+    for (int t = tb; t < te; ++t) {
+        const int jt = t * kTile;
+        for (int jg = 0; jg < kTile; jg += G) {
+            JRec J[G];
+            bool fl[G];
+            bool any = false;
+#pragma unroll
+            for (int k = 0; k < G; ++k) J[k] = jp[jt + jg + k];
+#pragma unroll
+            for (int k = 0; k < G; ++k) {
+                const double dx = J[k].x - x0;
+                const double dy = J[k].y - y0;
+                const double dz = J[k].z - z0;
+                const float fx = (float)dx, fy = (float)dy, fz = (float)dz;
+                const float gx = (float)(J[k].vx - u0);
+                const float gy = (float)(J[k].vy - w0);
+                const float gz = (float)(J[k].vz - q0);
+                const float r2 = fmaf(fz, fz, fmaf(fy, fy, fx * fx));
+                const float px = fmaf(dtr32, gx, fx);
+                const float py = fmaf(dtr32, gy, fy);
+                const float pz = fmaf(dtr32, gz, fz);
+                const float r2p = fmaf(pz, pz, fmaf(py, py, px * px));
+                float tj = thr;
+                if (MFLAG) tj = fmaxf(thr * J[k].m32, FLT_MIN);
+                fl[k] = fminf(r2, r2p) < tj;
+                any = any || fl[k];
+                const float rinv = fl[k] ? 0.0f : __builtin_amdgcn_rsqf(r2);
+                pair_terms(J[k], fx, fy, fz, gx, gy, gz, r2, rinv);
+            }
+            if (any) {
+#pragma unroll
+                for (int k = 0; k < G; ++k)
+                    if (fl[k]) resolve(J[k], jt + jg + k);
+            }
+        }
+        Ax += (double)ax;
+        Ay += (double)ay;
+        Az += (double)az;
+        Jx += (double)jx;
+        Jy += (double)jy;
+        Jz += (double)jz;
+        Ph += (double)ph;
+        ax = ay = az = jx = jy = jz = ph = 0.f;
+    }
+
+    if (act) {
+        double *p = part + ((size_t)i * nslice + s) * 8;
+        p[0] = Ax;
+        p[1] = Ay;
+        p[2] = Az;
+        p[3] = Jx;
+        p[4] = Jy;
+        p[5] = Jz;
+        p[6] = Ph;
+        p[7] = 0.0;
+        cnt[(size_t)i * nslice + s] = c;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Regular force, pass 2: one wavefront per i.  Sums the slice partials in a
+// fixed order (deterministic FP64), concatenates the per-slice neighbour
+// segments in slice order (= ascending j), finds the nearest neighbour
+// (smallest FP64 r2 > 0, ties to the smaller j) and writes the outputs.
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_regf_pass2(
+    const JRec *__restrict__ jp, const double *__restrict__ xi, int ni,
+    int nslice, int cap, int ntile, const double *__restrict__ part,
+    const int *__restrict__ cnt, const uint16_t *__restrict__ seg,
+    double *__restrict__ acc, double *__restrict__ jrk,
+    double *__restrict__ pot, int *__restrict__ nnb_out,
+    int *__restrict__ nnid, int *__restrict__ list, int *__restrict__ off,
+    int *__restrict__ total, long long list_cap, int nnbmax,
+    unsigned *__restrict__ err) {
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= ni) return;
+
+    double sum[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int q = lane; q < nslice; q += 64) {
+        const double *p = part + ((size_t)i * nslice + q) * 8;
+#pragma unroll
+        for (int k = 0; k < 7; ++k) sum[k] += p[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 7; ++k)
+        for (int o = 32; o >= 1; o >>= 1) sum[k] += __shfl_xor(sum[k], o);
+
+    const int *ci = cnt + (size_t)i * nslice;
+    int tot = 0;
+    for (int q0 = 0; q0 < nslice; q0 += 64) {
+        const int q = q0 + lane;
+        tot += wave_sum(q < nslice ? ci[q] : 0);
+    }
+
+    int o = -1;
+    int bestj = -1;
+    if (tot <= nnbmax) {
+        if (lane == 0) o = atomicAdd(total, tot);
+        o = __shfl(o, 0);
+        if ((long long)o + tot > list_cap) {
+            if (lane == 0) atomicOr(err, ERR_LIST_CAP);
+            o = -1;
+        } else {
+            const double xa = xi[3 * (size_t)i + 0];
+            const double ya = xi[3 * (size_t)i + 1];
+            const double za = xi[3 * (size_t)i + 2];
+            double br2 = INFINITY;
+            int bj = INT_MAX;
+            int base = o;
+            for (int q0 = 0; q0 < nslice; q0 += 64) {
+                const int q = q0 + lane;
+                const int cq = q < nslice ? ci[q] : 0;
+                const int incl = wave_incl_scan(cq, lane);
+                if (cq > 0) {
+                    const int js = (int)(((long long)ntile * q) / nslice) * kTile;
+                    const uint16_t *sq = seg + ((size_t)i * nslice + q) * cap;
+                    int *dst = list + base + (incl - cq);
+                    for (int k = 0; k < cq; ++k) {
+                        const int j = js + (int)sq[k];
+                        dst[k] = j;
+                        const JRec &J = jp[j];
+                        const double r2 = sq3d(J.x - xa, J.y - ya, J.z - za);
+                        if (r2 > 0.0 && (r2 < br2 || (r2 == br2 && j < bj))) {
+                            br2 = r2;
+                            bj = j;
+                        }
+                    }
+                }
+                base += __shfl(incl, 63);
+            }
+            // Lexicographic (r2, j) minimum over the wave.
+            for (int w = 32; w >= 1; w >>= 1) {
+                const double r2o = __shfl_xor(br2, w);
+                const int jo = __shfl_xor(bj, w);
+                if (r2o < br2 || (r2o == br2 && jo < bj)) {
+                    br2 = r2o;
+                    bj = jo;
+                }
+            }
+            bestj = (bj == INT_MAX) ? -1 : bj;
+        }
+    }
+    if (lane == 0) {
+        acc[3 * (size_t)i + 0] = sum[0];
+        acc[3 * (size_t)i + 1] = sum[1];
+        acc[3 * (size_t)i + 2] = sum[2];
+        jrk[3 * (size_t)i + 0] = sum[3];
+        jrk[3 * (size_t)i + 1] = sum[4];
+        jrk[3 * (size_t)i + 2] = sum[5];
+        pot[i] = sum[6];
+        nnb_out[i] = (tot <= nnbmax) ? tot : -tot;
+        off[i] = o;
+        if (nnid) nnid[i] = bestj;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Potential: phi_i = sum_{j: r2 > 0} m_j / r_ij.  FP64 differences, FP32
+// r2 / rsqrt / products, FP32 tile sums folded into FP64.
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_pack_p(const double *__restrict__ m,
+                                                const double *__restrict__ x,
+                                                int n, int n_pad,
+                                                PRec *__restrict__ p,
+                                                unsigned *__restrict__ err) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) {
+        PRec r;
+        r.x = x[3 * (size_t)j + 0];
+        r.y = x[3 * (size_t)j + 1];
+        r.z = x[3 * (size_t)j + 2];
+        const double mj = m[j];
+        r.m32 = (float)mj;
+        r.pad = 0.0f;
+        const double chk = r.x + r.y + r.z + mj;
+        if (chk != chk) atomicOr(err, ERR_NAN_J);
+        p[j] = r;
+    } else if (j < n_pad) {
+        PRec r;
+        r.x = r.y = r.z = kPadPos;
+        r.m32 = 0.0f;
+        r.pad = 0.0f;
+        p[j] = r;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_pot_pass1(const PRec *__restrict__ p,
+                                                  int ntile, int i0, int ni,
+                                                  int n_iw, int nslice,
+                                                  double *__restrict__ part) {
+    const int ntask = n_iw * nslice;
+    const int per = (ntask + 7) >> 3;
+    const int task = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    if (task >= ntask) return;
+    const int s = task / n_iw;
+    const int iw = task - s * n_iw;
+    const int q = iw * 64 + threadIdx.x;
+    const bool act = q < ni;
+    const int tb = (int)(((long long)ntile * s) / nslice);
+    const int te = (int)(((long long)ntile * (s + 1)) / nslice);
+    double x0 = 0.0, y0 = 0.0, z0 = 0.0;
+    if (act) {
+        const PRec r = p[i0 + q];
+        x0 = r.x;
+        y0 = r.y;
+        z0 = r.z;
+    }
+    float a = 0.f;
+    double A = 0.0;
+    for (int t = tb; t < te; ++t) {
+        const int jt = t * kTile;
+#pragma unroll 8
+        for (int jj = 0; jj < kTile; ++jj) {
+            const PRec J = p[jt + jj];
+            const float fx = (float)(J.x - x0);
+            const float fy = (float)(J.y - y0);
+            const float fz = (float)(J.z - z0);
+            const float r2 = fmaf(fz, fz, fmaf(fy, fy, fx * fx));
+            const float term = J.m32 * __builtin_amdgcn_rsqf(r2);
+            a += (r2 > 0.0f) ? term : 0.0f;
+        }
+        A += (double)a;
+        a = 0.f;
+    }
+    if (act) part[(size_t)q * nslice + s] = A;
+}
+
+__global__ __launch_bounds__(256) void k_pot_pass2(const double *__restrict__ part,
+                                                   int ni, int nslice,
+                                                   double *__restrict__ pot) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= ni) return;
+    double s = 0.0;
+    for (int k = 0; k < nslice; ++k) s += part[(size_t)q * nslice + k];
+    pot[q] = s;
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+
+static inline int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+void launch_stats_reset(hipStream_t st, JStats *jst, int *total) {
+    hipLaunchKernelGGL(k_stats_reset, dim3(1), dim3(64), 0, st, jst, total);
+}
+
+void launch_pack_j(hipStream_t st, const double *m, const double *x,
+                   const double *v, int nj, int nj_pad, JRec *jp, JStats *jst,
+                   unsigned *err) {
+    if (nj_pad <= 0) return;
+    hipLaunchKernelGGL(k_pack_j, dim3(cdiv(nj_pad, 256)), dim3(256), 0, st, m,
+                       x, v, nj, nj_pad, jp, jst, err);
+}
+
+void launch_predict(hipStream_t st, double t, int nj, int nj_pad,
+                    const double *x0, const double *v0, const double *f,
+                    const double *fd, const double *t0, const double *m,
+                    JRec *jp, JStats *jst) {
+    if (nj_pad <= 0) return;
+    hipLaunchKernelGGL(k_predict, dim3(cdiv(nj_pad, 256)), dim3(256), 0, st, t,
+                       nj, nj_pad, x0, v0, f, fd, t0, m, jp, jst);
+}
+
+void launch_jpset(hipStream_t st, int n, const int *addr, const double *src,
+                  double *x0, double *v0, double *f, double *fd, double *mass,
+                  double *t0, unsigned *err) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_jpset, dim3(cdiv(n, 256)), dim3(256), 0, st, n, addr,
+                       src, x0, v0, f, fd, mass, t0, err);
+}
+
+void launch_jget(hipStream_t st, int n, const JRec *jp, double *x, double *v) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_jget, dim3(cdiv(n, 256)), dim3(256), 0, st, n, jp, x,
+                       v);
+}
+
+void launch_zero_int(hipStream_t st, int *p, int n) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_zero_int, dim3(cdiv(n, 256)), dim3(256), 0, st, p, n);
+}
+
+template <bool MFLAG, int G>
+static void launch_pass1(hipStream_t st, const RegfLaunch &L, int n_iw,
+                         int nblk) {
+    hipLaunchKernelGGL((k_regf_pass1<MFLAG, G>), dim3(nblk), dim3(64), 0, st,
+                       L.jp, L.nj, L.ntile, L.jst, L.xi, L.vi, L.h2, L.dtr,
+                       L.ni, n_iw, L.nslice, L.cap, L.part, L.cnt, L.seg,
+                       L.err);
+}
+
+void launch_regf_pass1(hipStream_t st, const RegfLaunch &L) {
+    if (L.ni <= 0) return;
+    const int n_iw = cdiv(L.ni, 64);
+    const int ntask = n_iw * L.nslice;
+    const int nblk = 8 * ((ntask + 7) / 8);
+    if (L.group == 2) {
+        if (L.m_flag) launch_pass1<true, 2>(st, L, n_iw, nblk);
+        else launch_pass1<false, 2>(st, L, n_iw, nblk);
+    } else {
+        if (L.m_flag) launch_pass1<true, 4>(st, L, n_iw, nblk);
+        else launch_pass1<false, 4>(st, L, n_iw, nblk);
+    }
+}
+
+void launch_regf_pass2(hipStream_t st, const RegfLaunch &L) {
+    if (L.ni <= 0) return;
+    hipLaunchKernelGGL(k_regf_pass2, dim3(cdiv(L.ni, 4)), dim3(256), 0, st,
+                       L.jp, L.xi, L.ni, L.nslice, L.cap, L.ntile, L.part,
+                       L.cnt, L.seg, L.acc, L.jrk, L.pot, L.nnb, L.nnid,
+                       L.list, L.off, L.total, L.list_cap, L.nnbmax, L.err);
+}
+
+void launch_pack_p(hipStream_t st, const double *m, const double *x, int n,
+                   int n_pad, PRec *p, unsigned *err) {
+    if (n_pad <= 0) return;
+    hipLaunchKernelGGL(k_pack_p, dim3(cdiv(n_pad, 256)), dim3(256), 0, st, m, x,
+                       n, n_pad, p, err);
+}
+
+void launch_pot(hipStream_t st, const PRec *p, int n, int ntile, int i0,
+                int ni, int nslice, double *part, double *pot) {
+    (void)n;
+    if (ni <= 0) return;
+    const int n_iw = cdiv(ni, 64);
+    const int ntask = n_iw * nslice;
+    const int nblk = 8 * ((ntask + 7) / 8);
+    hipLaunchKernelGGL(k_pot_pass1, dim3(nblk), dim3(64), 0, st, p, ntile, i0,
+                       ni, n_iw, nslice, part);
+    hipLaunchKernelGGL(k_pot_pass2, dim3(cdiv(ni, 256)), dim3(256), 0, st, part,
+                       ni, nslice, pot);
+}
+
+static int resident_waves_of(const void *fn) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return 0;
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 64, 0) !=
+        hipSuccess)
+        return 0;
+    return nb * prop.multiProcessorCount;
+}
+
+int regf_resident_waves(int m_flag, int group) {
+    if (group == 2)
+        return m_flag ? resident_waves_of((const void *)k_regf_pass1<true, 2>)
+                      : resident_waves_of((const void *)k_regf_pass1<false, 2>);
+    return m_flag ? resident_waves_of((const void *)k_regf_pass1<true, 4>)
+                  : resident_waves_of((const void *)k_regf_pass1<false, 4>);
+}
+
+int pot_resident_waves() { return resident_waves_of((const void *)k_pot_pass1); }
+
+}  // namespace gpunb

@@ -1,0 +1,178 @@
+"""GPU parity at the benchmark configurations (Kroupa + binaries N=16384 in
+i-blocks of 1024/4096/16384; King W0=6 N=131072), the potential and the
+device-resident predictor, through the C-ABI.
+
+The oracle is exact FP64 O(ni * N); at N=16384 and N=131072 it checks a
+seeded sample of the i-particles of every call, and size-independent
+properties cover all of them.
+"""
+import numpy as np
+import pytest
+
+from nbody6ppgpu_amd import gpunb, ics
+from oracle import oracle
+from oracle.parity import FORCE_TOL, assert_parity
+from test_gpu_regf import setup
+
+pytestmark = pytest.mark.gpu
+
+
+def kroupa_binaries(n, seed):
+    c = ics.plummer(n, seed=seed, m=ics.kroupa(n, seed=seed + 1))
+    return ics.with_binaries(c, 0.1, seed=seed + 2)
+
+
+def check_global_properties(lst, ni):
+    cnt = lst[:, 0]
+    for i in range(ni):
+        k = cnt[i]
+        if k > 1:
+            assert np.all(np.diff(lst[i, 1:1 + k]) > 0), f"list {i} unsorted"
+
+
+@pytest.mark.parametrize("m_flag", [0, 1])
+def test_kroupa_binaries_16k_blocks(m_flag):
+    c = kroupa_binaries(16384, 201)
+    h2, dtr = setup(c, 48, m_flag)
+    rng = np.random.default_rng(202)
+    lmax, nnbmax = 601, 550
+    gpunb.gpunb_open(c.n + 10)
+    try:
+        gpunb.gpunb_send(c.m, c.x, c.v)
+        for nblk in (1024, 4096, c.n):
+            idx = np.sort(rng.choice(c.n, nblk, replace=False))
+            g = gpunb.gpunb_regf(h2[idx], dtr[idx], c.x[idx], c.v[idx], lmax,
+                                 nnbmax, m_flag, nearest=True)
+            check_global_properties(g[3], nblk)
+            sub = rng.choice(nblk, 384, replace=False)
+            o = oracle.regf(c.m, c.x, c.v, h2[idx][sub], dtr[idx][sub],
+                            c.x[idx][sub], c.v[idx][sub], lmax, nnbmax, m_flag)
+            assert_parity(tuple(a[sub] for a in g), o,
+                          label=f"16k block {nblk} m_flag={m_flag}",
+                          inputs=(c.m, c.x, c.v, c.x[idx][sub], c.v[idx][sub]))
+    finally:
+        gpunb.gpunb_close()
+
+
+def test_king_131072_full_sweep():
+    c = ics.king(131072, 6.0, seed=203, m=ics.kroupa(131072, seed=204))
+    h2, dtr = setup(c, 48)
+    lmax, nnbmax = 601, 550
+    gpunb.gpunb_open(c.n + 10)
+    try:
+        gpunb.gpunb_send(c.m, c.x, c.v)
+        g = gpunb.gpunb_regf(h2, dtr, c.x, c.v, lmax, nnbmax, 0, nearest=True)
+    finally:
+        gpunb.gpunb_close()
+    cnt = g[3][:, 0]
+    assert np.all(cnt >= 1)                      # self is always listed
+    rng = np.random.default_rng(205)
+    sub = np.sort(rng.choice(c.n, 256, replace=False))
+    # Include i from every internal launch chunk, first and last particles.
+    sub = np.unique(np.concatenate([sub, [0, 32767, 32768, 65536, c.n - 1]]))
+    check_global_properties(g[3][sub], len(sub))
+    for i in sub:
+        assert i in g[3][i, 1:1 + cnt[i]]
+    o = oracle.regf(c.m, c.x, c.v, h2[sub], dtr[sub], c.x[sub], c.v[sub],
+                    lmax, nnbmax, 0)
+    assert_parity(tuple(a[sub] for a in g), o, label="King 131072",
+                  inputs=(c.m, c.x, c.v, c.x[sub], c.v[sub]))
+
+
+def test_gpupot_parity_and_windows():
+    c = ics.plummer(4000, seed=206, m=ics.kroupa(4000, seed=207))
+    p_all = gpunb.gpupot(1, c.n, c.m, c.x)
+    o_all = oracle.pot(1, c.n, c.m, c.x)
+    err = np.abs(p_all - o_all) / o_all
+    print(f"gpupot max rel err {err.max():.2e}")
+    assert err.max() < FORCE_TOL
+    for istart, ni in ((1, 1), (100, 63), (3937, 64), (2, 3999)):
+        p = gpunb.gpupot(istart, ni, c.m, c.x)
+        o = oracle.pot(istart, ni, c.m, c.x)
+        assert np.max(np.abs(p - o) / o) < FORCE_TOL
+    # Closed form: regular tetrahedron of unit edge, phi = 3 m.
+    t = np.array([[1, 1, 1], [1, -1, -1], [-1, 1, -1], [-1, -1, 1]]) / np.sqrt(8)
+    assert np.allclose(gpunb.gpupot(1, 4, np.ones(4), t), 3.0, rtol=1e-6)
+
+
+def test_gpupot_131072_sample():
+    c = ics.king(131072, 6.0, seed=208, m=ics.kroupa(131072, seed=209))
+    p = gpunb.gpupot(1, c.n, c.m, c.x)
+    rng = np.random.default_rng(210)
+    for q in rng.choice(c.n, 64, replace=False):
+        o = oracle.pot(int(q) + 1, 1, c.m, c.x)[0]
+        assert abs(p[q] - o) / o < FORCE_TOL
+
+
+def test_device_predictor_bit_exact():
+    rng = np.random.default_rng(211)
+    n = 5000
+    c = ics.plummer(n, seed=212)
+    x0, v0 = c.x, c.v
+    f = rng.normal(0, 0.5, (n, 3))
+    fd = rng.normal(0, 0.5, (n, 3))
+    t0 = rng.uniform(0.0, 0.125, n)
+    t = 0.25
+    gpunb.gpunb_open(n + 10)
+    try:
+        addr = np.arange(1, n + 1)
+        # Two bulk updates, second one overwriting a subset out of order.
+        gpunb.gpunb_jpset(addr, x0 * 0.5, v0, f, fd, c.m, t0)
+        perm = rng.permutation(n)
+        gpunb.gpunb_jpset(addr[perm], x0[perm], v0[perm], f[perm], fd[perm],
+                          c.m[perm], t0[perm])
+        gpunb.gpunb_predict(t)
+        xg, vg = gpunb.gpunb_jpget(n)
+        xo, vo = oracle.predict(t, t0, x0, v0, f, fd)
+        assert np.array_equal(xg, xo) and np.array_equal(vg, vo)
+        # Regular force on the predicted j set == regular force on the same
+        # set sent from the host.
+        h2, dtr = setup(ics.Cluster(c.m, xo, vo), 40)
+        g1 = gpunb.gpunb_regf(h2, dtr, xo, vo, 301, 300, 0, nearest=True)
+        gpunb.gpunb_send(c.m, xo, vo)
+        g2 = gpunb.gpunb_regf(h2, dtr, xo, vo, 301, 300, 0, nearest=True)
+        for a, b in zip(g1, g2):
+            assert np.array_equal(a, b)
+    finally:
+        gpunb.gpunb_close()
+
+
+def test_device_entry_point_matches_abi():
+    from nbody6ppgpu_amd.hiprt import DeviceArray
+    c = ics.plummer(6000, seed=213)
+    h2, dtr = setup(c, 48)
+    ni = c.n
+    nnbmax = 500
+    gpunb.gpunb_open(c.n + 10)
+    try:
+        gpunb.gpunb_send(c.m, c.x, c.v)
+        ref = gpunb.gpunb_regf(h2, dtr, c.x, c.v, nnbmax + 1, nnbmax, 0,
+                               nearest=True)
+        dxi, dvi = DeviceArray.from_numpy(c.x), DeviceArray.from_numpy(c.v)
+        dh2, ddt = DeviceArray.from_numpy(h2), DeviceArray.from_numpy(dtr)
+        acc = DeviceArray((ni, 3), np.float64).zero()
+        jrk = DeviceArray((ni, 3), np.float64).zero()
+        pot = DeviceArray((ni,), np.float64).zero()
+        nnb = DeviceArray((ni,), np.int32).zero()
+        nnid = DeviceArray((ni,), np.int32).zero()
+        off = DeviceArray((ni,), np.int32).zero()
+        cap = ni * nnbmax
+        lst = DeviceArray((cap,), np.int32).zero()
+        tot = DeviceArray((1,), np.int32).zero()
+        rc = gpunb.load().gpunb_regf_dev(
+            ni, dh2.ptr, ddt.ptr, dxi.ptr, dvi.ptr, acc.ptr, jrk.ptr, pot.ptr,
+            nnbmax, 0, nnb.ptr, nnid.ptr, lst.ptr, off.ptr, tot.ptr, cap)
+        assert rc == 0
+        gpunb.gpunb_sync()
+        assert np.array_equal(acc.numpy(), ref[0])
+        assert np.array_equal(jrk.numpy(), ref[1])
+        assert np.array_equal(pot.numpy(), ref[2])
+        assert np.array_equal(nnb.numpy(), ref[3][:, 0])
+        assert np.array_equal(nnid.numpy(), ref[4])
+        L, O = lst.numpy(), off.numpy()
+        assert int(tot.numpy()[0]) == int(ref[3][:, 0].clip(0).sum())
+        for i in range(ni):
+            k = ref[3][i, 0]
+            assert np.array_equal(L[O[i]:O[i] + k], ref[3][i, 1:1 + k])
+    finally:
+        gpunb.gpunb_close()
