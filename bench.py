@@ -100,6 +100,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true",
                     help="skip the CPU baseline leg")
+    ap.add_argument("--prec", type=int, default=0,
+                    help="pair arithmetic: 32, 64 or 65 (0 = library default)")
     args = ap.parse_args()
     if args.gpus != 1:
         sys.exit("bench.py runs on one GPU only: this build measures the "
@@ -128,6 +130,8 @@ def main():
     tot = DeviceArray((1,), np.int32)
     lib = gpunb.load()
     stream = gpunb.gpunb_stream()
+    if args.prec:
+        gpunb.gpunb_set_precision(args.prec)
 
     def step():
         gpunb.gpunb_predict(0.0)

@@ -176,6 +176,12 @@ const char *gpunb_version(void);
  * Returns 0, or -1 with gpunb_last_error() set. */
 int gpunb_set_tuning(int group, double rounds);
 
+/* Pair arithmetic of the regular-force kernel: 32 = FP32 pair arithmetic on
+ * FP64 differences (default), 64 = FP64 pair arithmetic with the hardware
+ * 1/sqrt, 65 = FP64 with one Newton step on 1/sqrt; 0 = default.  Neighbour
+ * lists are identical in every mode.  Returns 0 or -1. */
+int gpunb_set_precision(int prec);
+
 /* Kernel timing: with enable != 0 the library records HIP events on its own
  * stream around every launch of the regular-force pair kernel; a read
  * synchronises and returns the summed kernel time (ms), the number of

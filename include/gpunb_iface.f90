@@ -1,0 +1,79 @@
+! gpunb_iface.f90 -- ISO_C_BINDING interfaces of the MI355X regular-force
+! library (include/gpunb.h).  Every procedure binds to the same symbol the
+! reference's Fortran host calls through implicit interfaces (lower case,
+! trailing underscore, all arguments by reference), so a caller may switch
+! between "CALL GPUNB_REGF(...)" (implicit, as util_gpu.F:60 does) and the
+! checked interface below without any change in the bytes passed.
+module gpunb_iface
+  use iso_c_binding, only: c_int, c_double
+  implicit none
+  interface
+    ! src/Main/gpunb.gpu.cu:822-824
+    subroutine gpunb_devinit(irank) bind(C, name="gpunb_devinit_")
+      import :: c_int
+      integer(c_int), intent(in) :: irank
+    end subroutine
+    ! src/Main/gpunb.gpu.cu:825-827 (caller intgrt.F:168, fpoly0.F:31)
+    subroutine gpunb_open(nbmax, irank) bind(C, name="gpunb_open_")
+      import :: c_int
+      integer(c_int), intent(in) :: nbmax, irank
+    end subroutine
+    ! src/Main/gpunb.gpu.cu:828-830 (caller fpoly0.F:375, intgrt.F:1497)
+    subroutine gpunb_close() bind(C, name="gpunb_close_")
+    end subroutine
+    ! src/Main/gpunb.gpu.cu:831-833 (caller intgrt.F:918, fpoly0.F:76)
+    subroutine gpunb_send(nj, mj, xj, vj) bind(C, name="gpunb_send_")
+      import :: c_int, c_double
+      integer(c_int), intent(in) :: nj
+      real(c_double), intent(in) :: mj(*), xj(3,*), vj(3,*)
+    end subroutine
+    ! src/Main/gpunb.gpu.cu:834-838 (caller util_gpu.F:60, fpoly0.F:125)
+    subroutine gpunb_regf(ni, h2, dtr, xi, vi, acc, jrk, pot, lmax, nnbmax, &
+                          list, m_flag) bind(C, name="gpunb_regf_")
+      import :: c_int, c_double
+      integer(c_int), intent(in) :: ni, lmax, nnbmax, m_flag
+      real(c_double), intent(in) :: h2(*), dtr(*), xi(3,*), vi(3,*)
+      real(c_double), intent(out) :: acc(3,*), jrk(3,*), pot(*)
+      integer(c_int), intent(out) :: list(lmax,*)
+    end subroutine
+    ! Additive: regf plus the nearest-neighbour index (IMINR, nbint.F:29-47).
+    subroutine gpunb_regfnn(ni, h2, dtr, xi, vi, acc, jrk, pot, lmax, nnbmax, &
+                            list, m_flag, nnid) bind(C, name="gpunb_regfnn_")
+      import :: c_int, c_double
+      integer(c_int), intent(in) :: ni, lmax, nnbmax, m_flag
+      real(c_double), intent(in) :: h2(*), dtr(*), xi(3,*), vi(3,*)
+      real(c_double), intent(out) :: acc(3,*), jrk(3,*), pot(*)
+      integer(c_int), intent(out) :: list(lmax,*), nnid(*)
+    end subroutine
+    ! src/Main/gpunb.gpu.cu:839-841 (caller adjust.F:315)
+    subroutine gpunb_profile(irank) bind(C, name="gpunb_profile_")
+      import :: c_int
+      integer(c_int), intent(in) :: irank
+    end subroutine
+    ! src/Main/gpupot.gpu.cu:116-127 (caller energy.F:38)
+    subroutine gpupot(irank, istart, ni, n, m, x, pot) bind(C, name="gpupot_")
+      import :: c_int, c_double
+      integer(c_int), intent(in) :: irank, istart, ni, n
+      real(c_double), intent(in) :: m(*), x(3,*)
+      real(c_double), intent(out) :: pot(*)
+    end subroutine
+    ! Additive: device-resident predictor state (irr_simd_set_jp_ in bulk).
+    subroutine gpunb_jpset(n, addr, x0, v0, f, fdot, mass, t0) &
+        bind(C, name="gpunb_jpset_")
+      import :: c_int, c_double
+      integer(c_int), intent(in) :: n, addr(*)
+      real(c_double), intent(in) :: x0(3,*), v0(3,*), f(3,*), fdot(3,*)
+      real(c_double), intent(in) :: mass(*), t0(*)
+    end subroutine
+    ! Additive: predict all stored j to time (xbpredall.f:18-26 on the GPU).
+    subroutine gpunb_predict(time) bind(C, name="gpunb_predict_")
+      import :: c_double
+      real(c_double), intent(in) :: time
+    end subroutine
+    subroutine gpunb_jpget(n, x, v) bind(C, name="gpunb_jpget_")
+      import :: c_int, c_double
+      integer(c_int), intent(in) :: n
+      real(c_double), intent(out) :: x(3,*), v(3,*)
+    end subroutine
+  end interface
+end module gpunb_iface

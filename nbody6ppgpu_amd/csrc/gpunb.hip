@@ -132,7 +132,7 @@ struct State {
     HostBuf<int> h_list;
     long long list_est = 0;
 
-    DevBuf<PRec> prec;
+    DevBuf<PRec> pot_rec;
     DevBuf<double> d_pin, d_ppart, d_pout;
     HostBuf<double> h_pin, h_pout;
 
@@ -147,10 +147,11 @@ struct State {
     std::vector<std::pair<int, int>> ev_pairs;   // indices into ev_pool
     std::vector<long long> ev_pairs_work;        // pair interactions per launch
 
-    int resident[2][2] = {{0, 0}, {0, 0}};
+    int resident[2][2][3] = {};
     int pot_resident = 0;
     int group = 0;          // 0: choose automatically
     double rounds = 0.0;    // 0: default
+    int prec = 0;           // 0: default pair arithmetic
 };
 
 State S;
@@ -261,9 +262,16 @@ double pick_rounds() {
     return 2.0;
 }
 
-int resident_waves(int m_flag, int group) {
-    int &r = S.resident[m_flag ? 1 : 0][group == 2 ? 0 : 1];
-    if (!r) r = regf_resident_waves(m_flag, group);
+int pick_prec() {
+    if (S.prec) return S.prec;
+    const char *p = getenv("GPUNB_PREC");
+    if (p && (atoi(p) == 32 || atoi(p) == 64 || atoi(p) == 65)) return atoi(p);
+    return 32;
+}
+
+int resident_waves(int m_flag, int group, int prec) {
+    int &r = S.resident[m_flag ? 1 : 0][group == 2 ? 0 : 1][prec >= 64 ? 1 : 0];
+    if (!r) r = regf_resident_waves(m_flag, group, prec);
     if (r <= 0) r = 256 * 8;
     return r;
 }
@@ -276,10 +284,11 @@ struct Decomp {
 // j slices per i-wavefront: enough waves to fill the resident capacity
 // `rounds` times; slices hold whole tiles, at most 1024 tiles (uint16 local
 // neighbour indices).
-Decomp decompose(int ni_c, int ntile, int nnbmax, int m_flag, int group) {
+Decomp decompose(int ni_c, int ntile, int nnbmax, int m_flag, int group,
+                 int prec) {
     Decomp d;
     const int n_iw = cdiv(ni_c, 64);
-    const double target = resident_waves(m_flag, group) * pick_rounds();
+    const double target = resident_waves(m_flag, group, prec) * pick_rounds();
     int ns = (int)std::lround(target / n_iw);
     ns = std::max(ns, 1);
     ns = std::min(ns, std::max(ntile, 1));
@@ -315,10 +324,11 @@ void enqueue_regf(int ni, const double *xi, const double *vi, const double *h2,
                   int nnbmax, int m_flag, int *nnb, int *nnid, int *list,
                   int *off, int *total, long long list_cap) {
     const int group = pick_group();
+    const int prec = pick_prec();
     const int ntile = cdiv(S.nj, kTile);
     for (int c0 = 0; c0 < ni; c0 += kChunk) {
         const int nc = std::min(kChunk, ni - c0);
-        Decomp d = decompose(nc, ntile, nnbmax, m_flag, group);
+        Decomp d = decompose(nc, ntile, nnbmax, m_flag, group, prec);
         ensure_regf_scratch(nc, d);
         RegfLaunch L;
         L.jp = S.jp.p;
@@ -347,6 +357,7 @@ void enqueue_regf(int ni, const double *xi, const double *vi, const double *h2,
         L.nnbmax = nnbmax;
         L.m_flag = m_flag;
         L.group = group;
+        L.prec = prec;
         L.err = S.err.p;
         if (ntile == 0) {
             // No j-particles: one empty slice, zero forces, empty lists.
@@ -615,12 +626,12 @@ void do_pot(int irank, int istart, int ni, int n, const double *m,
     const int n_pad = ntile * kTile;
     S.h_pin.ensure(4 * (size_t)n);
     S.d_pin.ensure(4 * (size_t)n);
-    S.prec.ensure((size_t)n_pad + kTile);
+    S.pot_rec.ensure((size_t)n_pad + kTile);
     memcpy(S.h_pin.p, m, (size_t)n * sizeof(double));
     memcpy(S.h_pin.p + n, x, 3 * (size_t)n * sizeof(double));
     HIPCHK(hipMemcpyAsync(S.d_pin.p, S.h_pin.p, 4 * (size_t)n * sizeof(double),
                           hipMemcpyHostToDevice, S.st));
-    launch_pack_p(S.st, S.d_pin.p, S.d_pin.p + n, n, n_pad + kTile, S.prec.p,
+    launch_pack_p(S.st, S.d_pin.p, S.d_pin.p + n, n, n_pad + kTile, S.pot_rec.p,
                   S.err.p);
     if (!S.pot_resident) S.pot_resident = pot_resident_waves();
     const int target = std::max(S.pot_resident, 1024) * 2;
@@ -630,7 +641,7 @@ void do_pot(int irank, int istart, int ni, int n, const double *m,
     S.d_ppart.ensure((size_t)round_up(ni, 64) * ns);
     S.d_pout.ensure((size_t)ni);
     S.h_pout.ensure((size_t)ni);
-    launch_pot(S.st, S.prec.p, n, ntile, istart - 1, ni, ns, S.d_ppart.p,
+    launch_pot(S.st, S.pot_rec.p, n, ntile, istart - 1, ni, ns, S.d_ppart.p,
                S.d_pout.p);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(S.h_pout.p, S.d_pout.p, (size_t)ni * sizeof(double),
@@ -877,6 +888,14 @@ GPUNB_EXPORT int gpunb_set_tuning(int group, double rounds) {
         REQUIRE(rounds >= 0.0, "gpunb_set_tuning: rounds must be >= 0");
         S.group = group;
         S.rounds = rounds;
+    });
+}
+
+GPUNB_EXPORT int gpunb_set_precision(int prec) {
+    return guarded("gpunb_set_precision", [&] {
+        REQUIRE(prec == 0 || prec == 32 || prec == 64 || prec == 65,
+                "gpunb_set_precision: prec must be 0, 32, 64 or 65");
+        S.prec = prec;
     });
 }
 

@@ -74,6 +74,7 @@ struct RegfLaunch {
     int nnbmax;
     int m_flag;
     int group;       // pairs per branch-free group of pass 1 (2 or 4)
+    int prec;        // 32: FP32 pair arithmetic; 64/65: FP64 (+ Newton steps)
     unsigned *err;
 };
 
@@ -99,7 +100,7 @@ void launch_pack_p(hipStream_t st, const double *m, const double *x, int n,
                    int n_pad, PRec *p, unsigned *err);
 
 // Occupancy-derived resident-wave capacity of the regular-force kernel.
-int regf_resident_waves(int m_flag, int group);
+int regf_resident_waves(int m_flag, int group, int prec);
 int pot_resident_waves();
 
 }  // namespace gpunb

