@@ -16,6 +16,7 @@
 //   k_pot_pass1/2  O(N^2) potential (gpupot.gpu.cu:32-57).
 #include "gpunb_internal.h"
 
+#include <algorithm>
 #include <cfloat>
 #include <climits>
 #include <cmath>
@@ -590,8 +591,9 @@ __global__ __launch_bounds__(64) void k_regf_pass1_f64(
 }
 
 // ---------------------------------------------------------------------------
-// Regular force, pass 2: a workgroup of 4 wavefronts serves kP2I i-particles,
-// each wavefront kP2I/4 of them in turn.  Per i: the slice partials are summed
+// Regular force, pass 2: a workgroup of 4 wavefronts serves ipb <= kP2I
+// i-particles (ipb chosen per launch so that small calls still spread over
+// the machine), each wavefront every 4th of them.  Per i: the slice partials are summed
 // in a fixed order (deterministic FP64) and the forces written; then one
 // thread scans the block's list lengths and claims the block's list space
 // with a single atomic; then every i's per-slice neighbour segments are
@@ -608,15 +610,15 @@ __global__ __launch_bounds__(256) void k_regf_pass2(
     double *__restrict__ acc, double *__restrict__ jrk,
     double *__restrict__ pot, int *__restrict__ nnb_out,
     int *__restrict__ nnid, int *__restrict__ list, int *__restrict__ off,
-    int *__restrict__ total, long long list_cap, int nnbmax,
+    int *__restrict__ total, long long list_cap, int nnbmax, int ipb,
     unsigned *__restrict__ err) {
     __shared__ int s_tot[kP2I];
     __shared__ int s_off[kP2I];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int i0 = blockIdx.x * kP2I;
+    const int i0 = blockIdx.x * ipb;
 
-    for (int r = wave; r < kP2I; r += 4) {
+    for (int r = wave; r < ipb; r += 4) {
         const int i = i0 + r;
         int tot = 0;
         if (i < ni) {
@@ -651,7 +653,7 @@ __global__ __launch_bounds__(256) void k_regf_pass2(
     __syncthreads();
     if (threadIdx.x == 0) {
         int run = 0;
-        for (int r = 0; r < kP2I; ++r) {
+        for (int r = 0; r < ipb; ++r) {
             const int t = s_tot[r];
             s_off[r] = run;
             if (t <= nnbmax) run += t;
@@ -661,12 +663,12 @@ __global__ __launch_bounds__(256) void k_regf_pass2(
             atomicOr(err, ERR_LIST_CAP);
             base = -1;
         }
-        for (int r = 0; r < kP2I; ++r)
+        for (int r = 0; r < ipb; ++r)
             s_off[r] = (base < 0 || s_tot[r] > nnbmax) ? -1 : base + s_off[r];
     }
     __syncthreads();
 
-    for (int r = wave; r < kP2I; r += 4) {
+    for (int r = wave; r < ipb; r += 4) {
         const int i = i0 + r;
         if (i >= ni) break;
         const int o = s_off[r];
@@ -895,10 +897,13 @@ void launch_regf_pass1(hipStream_t st, const RegfLaunch &L) {
 
 void launch_regf_pass2(hipStream_t st, const RegfLaunch &L) {
     if (L.ni <= 0) return;
-    hipLaunchKernelGGL(k_regf_pass2, dim3(cdiv(L.ni, kP2I)), dim3(256), 0, st,
+    // About 2048 workgroups for large calls (one list-space atomic per 16 i),
+    // one i per wavefront for small ones.
+    const int ipb = std::max(4, std::min(kP2I, L.ni / 2048));
+    hipLaunchKernelGGL(k_regf_pass2, dim3(cdiv(L.ni, ipb)), dim3(256), 0, st,
                        L.jp, L.xi, L.ni, L.nslice, L.cap, L.ntile, L.part,
                        L.cnt, L.seg, L.acc, L.jrk, L.pot, L.nnb, L.nnid,
-                       L.list, L.off, L.total, L.list_cap, L.nnbmax, L.err);
+                       L.list, L.off, L.total, L.list_cap, L.nnbmax, ipb, L.err);
 }
 
 void launch_pack_p(hipStream_t st, const double *m, const double *x, int n,
