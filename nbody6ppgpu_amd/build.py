@@ -47,5 +47,25 @@ def build_lib(verbose: bool = False) -> str:
     return LIB
 
 
+HOSTDIR = os.path.join(HERE, "host")
+BINDIR = os.path.join(HERE, "bin")
+HERMITE = os.path.join(BINDIR, "hermite_ac")
+
+
+def build_host(verbose: bool = False) -> str:
+    """The host Hermite/Ahmad-Cohen driver (C++, OpenMP) linked to the
+    library with an rpath relative to itself."""
+    os.makedirs(BINDIR, exist_ok=True)
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-fopenmp",
+           "-Wall", "-I" + os.path.join(ROOT, "include"),
+           os.path.join(HOSTDIR, "hermite_ac.cpp"), "-L" + LIBDIR, "-lgpunb",
+           "-Wl,-rpath,$ORIGIN/../lib", "-o", HERMITE]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+    return HERMITE
+
+
 if __name__ == "__main__":
     print(build_lib(verbose=True))
+    print(build_host(verbose=True))

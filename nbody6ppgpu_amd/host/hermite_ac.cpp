@@ -1,0 +1,724 @@
+// hermite_ac.cpp -- a host Hermite / Ahmad-Cohen block-step integrator that
+// drives the regular-force engine through its C-ABI the way the reference's
+// Fortran host does (intgrt.F: predict all j, send, regular force for the
+// particles whose regular step is due, irregular force from the returned
+// neighbour lists, correct, new steps; energy.F: gpupot at check times).
+//
+// It exists to measure what the engine must sustain in a real run: pair
+// interactions per second over a whole integration and the relative energy
+// error |dE/E| over N-body time units.  It is a compact restatement of the
+// fourth-order Hermite Ahmad-Cohen scheme (Makino & Aarseth 1992, PASJ 44,
+// 141), not a port of NBODY6++: no KS or chain regularisation, no stellar
+// evolution, no tidal field.
+//
+// Per particle i the force is split in an irregular part (neighbours in the
+// list L_i, summed here in FP64 every irregular step dt_i) and a regular part
+// (all other particles, from gpunb_regf_ every regular step dtr_i, extended
+// between regular steps by its Taylor polynomial).  Hermite interpolation of
+// two force/jerk pairs gives the second and third derivatives for the
+// corrector; when a regular step changes the list, the regular force seen
+// with the old list is recovered as F_reg(L') + F_irr(L') - F_irr(L), so the
+// regular polynomial stays that of one consistent split (regcor_gpu.F's
+// gain/loss bookkeeping in its simplest form).  Steps follow Aarseth's
+// criterion with ETAI / ETAR, quantised to powers of two and kept
+// commensurate (block steps); the neighbour radius is adjusted towards NNBOPT
+// neighbours at every regular step, and an overflowing list is retried with a
+// smaller radius (util_gpu.F:70-101).
+//
+// Usage:  hermite_ac <in.bin> <t_end> [options]   (see parse_args)
+// in.bin: int32 n, then FP64 m[n], x[n][3], v[n][3].
+// Output: one JSON object per energy check on stdout.
+#include "gpunb.h"
+
+#include <omp.h>
+#include <sys/time.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace {
+
+double wtime() {
+    struct timeval tv;
+    gettimeofday(&tv, nullptr);
+    return tv.tv_sec + 1e-6 * tv.tv_usec;
+}
+
+struct Opts {
+    double t_end = 1.0;
+    double etai = 0.05;
+    double etar = 0.1;
+    double dtadj = 0.125;      // energy check interval (power of two)
+    double dtmin = 1.0 / (1 << 30);
+    int nnbopt = 48;
+    int lmax = 601;            // LMAX (params.h), list row stride
+    int nnbmax = 550;          // NNBMAX = min(N/2, LMAX-50) (input.F:121)
+    int use_predict = 1;       // 1: device predictor; 0: host predict + send
+    int prec = 0;              // gpunb_set_precision
+    double rs0 = 0.0;          // initial RS (0: from nearest neighbours)
+};
+
+struct Body {
+    double m;
+    double x0[3], v0[3];       // state at t0
+    double a[3], j[3];         // total force and jerk at t0
+    double fi[3], fid[3];      // irregular force, jerk at t0 (list L)
+    double fi2[3], fi3[3];     // irregular 2nd/3rd derivatives at t0
+    double fr[3], frd[3], fr2[3], fr3[3];  // regular polynomial at t0r
+    double t0, dt, t0r, dtr;
+    double rs;
+    std::vector<int> list;     // neighbours (0-based, self excluded)
+};
+
+inline double norm3(const double *a) { return std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]); }
+
+// Largest power of two <= x (x > 0).
+double pow2_floor(double x) {
+    int e;
+    double f = std::frexp(x, &e);  // x = f 2^e, f in [0.5, 1)
+    (void)f;
+    return std::ldexp(1.0, e - 1);
+}
+
+// Aarseth's step criterion.
+double aarseth(const double *f, const double *f1, const double *f2,
+               const double *f3, double eta) {
+    const double a0 = norm3(f), a1 = norm3(f1), a2 = norm3(f2), a3 = norm3(f3);
+    const double num = a0 * a2 + a1 * a1;
+    const double den = a1 * a3 + a2 * a2;
+    if (den <= 0.0 || num <= 0.0) return 1e30;
+    return std::sqrt(eta * num / den);
+}
+
+// Predict body b to time t (cubic, the order the engine's predictor uses).
+inline void predict_cubic(const Body &b, double t, double *x, double *v) {
+    const double s = t - b.t0;
+    for (int k = 0; k < 3; ++k) {
+        x[k] = ((b.j[k] * (s / 6.0) + 0.5 * b.a[k]) * s + b.v0[k]) * s + b.x0[k];
+        v[k] = (b.j[k] * (0.5 * s) + b.a[k]) * s + b.v0[k];
+    }
+}
+
+// Force and jerk on (xi, vi) from the listed bodies predicted to t (FP64).
+void irr_force(const std::vector<Body> &B, const std::vector<int> &list,
+               double t, const double *xi, const double *vi, double *f,
+               double *fd) {
+    f[0] = f[1] = f[2] = fd[0] = fd[1] = fd[2] = 0.0;
+    for (int jj : list) {
+        double xj[3], vj[3];
+        predict_cubic(B[jj], t, xj, vj);
+        const double dx = xj[0] - xi[0], dy = xj[1] - xi[1], dz = xj[2] - xi[2];
+        const double ux = vj[0] - vi[0], uy = vj[1] - vi[1], uz = vj[2] - vi[2];
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 == 0.0) continue;
+        const double rinv2 = 1.0 / r2;
+        const double mr3 = B[jj].m * rinv2 * std::sqrt(rinv2);
+        const double al = 3.0 * (dx * ux + dy * uy + dz * uz) * rinv2;
+        f[0] += mr3 * dx;
+        f[1] += mr3 * dy;
+        f[2] += mr3 * dz;
+        fd[0] += mr3 * (ux - al * dx);
+        fd[1] += mr3 * (uy - al * dy);
+        fd[2] += mr3 * (uz - al * dz);
+    }
+}
+
+// Second and third time derivatives of the force m r/|r|^3 that body j
+// (relative position r, velocity v, acceleration a, jerk jk) exerts
+// (Nitadori & Makino 2008, eqs. for the sixth-order Hermite scheme).
+inline void pair_d23(double m, const double *r, const double *v,
+                     const double *a, const double *jk, double *f2,
+                     double *f3) {
+    const double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+    const double ri2 = 1.0 / r2;
+    const double rv = r[0] * v[0] + r[1] * v[1] + r[2] * v[2];
+    const double vv = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    const double ra = r[0] * a[0] + r[1] * a[1] + r[2] * a[2];
+    const double va = v[0] * a[0] + v[1] * a[1] + v[2] * a[2];
+    const double rj = r[0] * jk[0] + r[1] * jk[1] + r[2] * jk[2];
+    const double al = rv * ri2;
+    const double be = (vv + ra) * ri2 + al * al;
+    const double ga = (3.0 * va + rj) * ri2 + al * (3.0 * be - 4.0 * al * al);
+    const double mr3 = m * ri2 * std::sqrt(ri2);
+    for (int k = 0; k < 3; ++k) {
+        const double F0 = mr3 * r[k];
+        const double F1 = mr3 * v[k] - 3.0 * al * F0;
+        const double F2 = mr3 * a[k] - 6.0 * al * F1 - 3.0 * be * F0;
+        f2[k] = F2;
+        f3[k] = mr3 * jk[k] - 9.0 * al * F2 - 9.0 * be * F1 - 3.0 * ga * F0;
+    }
+}
+
+// Hermite interpolation: derivatives 2 and 3 at the start of an interval of
+// length h from (f0, f0') and (f1, f1').
+inline void hermite23(const double *f0, const double *d0, const double *f1,
+                      const double *d1, double h, double *f2, double *f3) {
+    const double h2 = h * h;
+    for (int k = 0; k < 3; ++k) {
+        f2[k] = (-6.0 * (f0[k] - f1[k]) - h * (4.0 * d0[k] + 2.0 * d1[k])) / h2;
+        f3[k] = (12.0 * (f0[k] - f1[k]) + 6.0 * h * (d0[k] + d1[k])) / (h2 * h);
+    }
+}
+
+struct Run {
+    Opts o;
+    int n = 0;
+    std::vector<Body> B;
+    double t = 0.0;
+    // work counters
+    long long reg_pairs = 0, irr_pairs = 0, reg_calls = 0, irr_steps = 0,
+              reg_steps = 0, blocks = 0, overflows = 0;
+    double t_reg = 0.0, t_irr = 0.0, t_pred = 0.0, t_pot = 0.0;
+    // staging for the ABI
+    std::vector<double> h2, dtrv, xi, vi, acc, jrk, pot, jm, jx, jv;
+    std::vector<int> lst;
+
+    void abi_check() {
+        char buf[512];
+        if (gpunb_last_error(buf, sizeof buf) > 0) {
+            fprintf(stderr, "hermite_ac: library error: %s\n", buf);
+            exit(2);
+        }
+    }
+
+    // Positions/velocities of every body at time t on the device: either
+    // the device predictor (state sent with gpunb_jpset_ after each
+    // correction) or a host prediction sent with gpunb_send_.
+    void send_j(double tt) {
+        double w0 = wtime();
+        if (o.use_predict) {
+            gpunb_predict_(&tt);
+        } else {
+            jm.resize(n);
+            jx.resize(3 * (size_t)n);
+            jv.resize(3 * (size_t)n);
+#pragma omp parallel for schedule(static)
+            for (int q = 0; q < n; ++q) {
+                jm[q] = B[q].m;
+                predict_cubic(B[q], tt, &jx[3 * (size_t)q], &jv[3 * (size_t)q]);
+            }
+            gpunb_send_(&n, jm.data(), (double(*)[3])jx.data(),
+                        (double(*)[3])jv.data());
+        }
+        abi_check();
+        t_pred += wtime() - w0;
+    }
+
+    void jpset(const std::vector<int> &idx) {
+        if (!o.use_predict || idx.empty()) return;
+        const int k = (int)idx.size();
+        std::vector<int> addr(k);
+        std::vector<double> X(3 * k), V(3 * k), F(3 * k), FD(3 * k), M(k), T(k);
+        for (int q = 0; q < k; ++q) {
+            const Body &b = B[idx[q]];
+            addr[q] = idx[q] + 1;
+            for (int c = 0; c < 3; ++c) {
+                X[3 * q + c] = b.x0[c];
+                V[3 * q + c] = b.v0[c];
+                F[3 * q + c] = 0.5 * b.a[c];
+                FD[3 * q + c] = b.j[c] / 6.0;
+            }
+            M[q] = b.m;
+            T[q] = b.t0;
+        }
+        int kk = k;
+        gpunb_jpset_(&kk, addr.data(), (double(*)[3])X.data(),
+                     (double(*)[3])V.data(), (double(*)[3])F.data(),
+                     (double(*)[3])FD.data(), M.data(), T.data());
+        abi_check();
+    }
+
+    // Regular force + new lists for bodies R at their positions (x, v).
+    // Retries overflowing lists with a smaller radius (util_gpu.F:70-101).
+    void regular(const std::vector<int> &R, const std::vector<double> &X,
+                 const std::vector<double> &V, std::vector<double> &FR,
+                 std::vector<double> &FRD, std::vector<std::vector<int>> &L) {
+        const int nr = (int)R.size();
+        FR.assign(3 * (size_t)nr, 0.0);
+        FRD.assign(3 * (size_t)nr, 0.0);
+        L.assign(nr, {});
+        std::vector<int> todo(nr);
+        for (int q = 0; q < nr; ++q) todo[q] = q;
+        while (!todo.empty()) {
+            int ni = (int)todo.size();
+            h2.resize(ni);
+            dtrv.resize(ni);
+            xi.resize(3 * (size_t)ni);
+            vi.resize(3 * (size_t)ni);
+            acc.resize(3 * (size_t)ni);
+            jrk.resize(3 * (size_t)ni);
+            pot.resize(ni);
+            lst.resize((size_t)ni * o.lmax);
+            for (int q = 0; q < ni; ++q) {
+                const int r = todo[q];
+                const Body &b = B[R[r]];
+                h2[q] = b.rs * b.rs;
+                dtrv[q] = b.dtr;
+                for (int c = 0; c < 3; ++c) {
+                    xi[3 * q + c] = X[3 * (size_t)r + c];
+                    vi[3 * q + c] = V[3 * (size_t)r + c];
+                }
+            }
+            double w0 = wtime();
+            int mflag = 0;
+            gpunb_regf_(&ni, h2.data(), dtrv.data(), (double(*)[3])xi.data(),
+                        (double(*)[3])vi.data(), (double(*)[3])acc.data(),
+                        (double(*)[3])jrk.data(), pot.data(), &o.lmax,
+                        &o.nnbmax, lst.data(), &mflag);
+            abi_check();
+            t_reg += wtime() - w0;
+            reg_calls++;
+            reg_pairs += (long long)ni * n;
+            std::vector<int> again;
+            for (int q = 0; q < ni; ++q) {
+                const int r = todo[q];
+                Body &b = B[R[r]];
+                const int *row = &lst[(size_t)q * o.lmax];
+                const int nnb = row[0];
+                if (nnb < 0) {
+                    // Overflow: shrink RS (NNBOPT/NNB)^(1/3) and redo.
+                    overflows++;
+                    b.rs *= std::pow((double)o.nnbopt / (double)(-nnb), 1.0 / 3.0);
+                    again.push_back(r);
+                    continue;
+                }
+                L[r].clear();
+                for (int k = 1; k <= nnb; ++k)
+                    if (row[k] != R[r]) L[r].push_back(row[k]);
+                for (int c = 0; c < 3; ++c) {
+                    FR[3 * (size_t)r + c] = acc[3 * q + c];
+                    FRD[3 * (size_t)r + c] = jrk[3 * q + c];
+                }
+            }
+            todo.swap(again);
+        }
+    }
+
+    // New irregular step for b at time tt from its irregular derivatives.
+    double new_dt(const Body &b, double tt, double dt_old, double cap) {
+        double f2[3], f3[3];
+        for (int k = 0; k < 3; ++k) {
+            f2[k] = b.fi2[k];
+            f3[k] = b.fi3[k];
+        }
+        double d = b.list.empty() ? cap : aarseth(b.fi, b.fid, f2, f3, o.etai);
+        d = std::min(d, 2.0 * dt_old);
+        d = std::min(d, cap);
+        d = std::max(d, o.dtmin);
+        double q = pow2_floor(d);
+        while (std::fmod(tt, q) != 0.0 && q > o.dtmin) q *= 0.5;
+        return q;
+    }
+
+    double new_dtr(const Body &b, double tt, double dtr_old, double dt) {
+        double d = aarseth(b.fr, b.frd, b.fr2, b.fr3, o.etar);
+        d = std::min(d, 2.0 * dtr_old);
+        d = std::min(d, o.dtadj);
+        double q = pow2_floor(std::max(d, dt));
+        while (std::fmod(tt, q) != 0.0 && q > dt) q *= 0.5;
+        return std::max(q, dt);
+    }
+
+    void init(const std::vector<double> &m, const std::vector<double> &x,
+              const std::vector<double> &v) {
+        B.resize(n);
+        for (int q = 0; q < n; ++q) {
+            Body &b = B[q];
+            b = Body{};
+            b.m = m[q];
+            for (int c = 0; c < 3; ++c) {
+                b.x0[c] = x[3 * (size_t)q + c];
+                b.v0[c] = v[3 * (size_t)q + c];
+            }
+            const double r = norm3(b.x0);
+            // FPOLY0 start values (fpoly0.F:52-55): RS0 sqrt(1 + r^2),
+            // STEPR = min(SMAX/8 sqrt(1 + r^2), SMAX).
+            b.rs = o.rs0 * std::sqrt(1.0 + r * r);
+            b.dtr = std::min(o.dtadj / 8.0 * std::sqrt(1.0 + r * r), o.dtadj);
+        }
+        // Everybody at t = 0: regular force and lists for all.
+        std::vector<int> R(n);
+        for (int q = 0; q < n; ++q) R[q] = q;
+        std::vector<double> X(3 * (size_t)n), V(3 * (size_t)n), FR, FRD;
+        for (int q = 0; q < n; ++q)
+            for (int c = 0; c < 3; ++c) {
+                X[3 * (size_t)q + c] = B[q].x0[c];
+                V[3 * (size_t)q + c] = B[q].v0[c];
+            }
+        std::vector<int> all(n);
+        for (int q = 0; q < n; ++q) all[q] = q;
+        jpset(all);   // device predictor state (a = j = 0 until the first force)
+        send_j(0.0);
+        std::vector<std::vector<int>> L;
+        regular(R, X, V, FR, FRD, L);
+        double w0 = wtime();
+#pragma omp parallel for schedule(dynamic, 64)
+        for (int q = 0; q < n; ++q) {
+            Body &b = B[q];
+            b.list = L[q];
+            irr_force(B, b.list, 0.0, b.x0, b.v0, b.fi, b.fid);
+            for (int c = 0; c < 3; ++c) {
+                b.fr[c] = FR[3 * (size_t)q + c];
+                b.frd[c] = FRD[3 * (size_t)q + c];
+                b.a[c] = b.fi[c] + b.fr[c];
+                b.j[c] = b.fid[c] + b.frd[c];
+            }
+            // Start with a conservative second-order step.
+            const double a0 = norm3(b.a), a1 = norm3(b.j);
+            double d = (a1 > 0) ? 0.1 * o.etai * a0 / a1 : o.dtadj;
+            const double ai = norm3(b.fi), ai1 = norm3(b.fid);
+            if (!b.list.empty() && ai1 > 0) d = std::min(d, 0.1 * o.etai * ai / ai1);
+            d = std::min(std::max(d, o.dtmin), o.dtadj / 8.0);
+            b.dt = pow2_floor(d);
+            double dr = (norm3(b.frd) > 0) ? 0.1 * o.etar * norm3(b.fr) / norm3(b.frd) : o.dtadj;
+            dr = std::min(std::max(dr, b.dt), o.dtadj);
+            b.dtr = std::max(pow2_floor(dr), b.dt);
+            b.t0 = 0.0;
+            b.t0r = 0.0;
+        }
+        t_irr += wtime() - w0;
+        irr_pairs += 0;
+        for (int q = 0; q < n; ++q) irr_pairs += (long long)B[q].list.size();
+        jpset(all);
+    }
+
+    double energy(double *ekin, double *epot) {
+        // Called at a synchronisation time: every body has t0 == t.
+        std::vector<double> m(n), x(3 * (size_t)n), ph(n);
+        double ke = 0.0;
+        for (int q = 0; q < n; ++q) {
+            m[q] = B[q].m;
+            for (int c = 0; c < 3; ++c) x[3 * (size_t)q + c] = B[q].x0[c];
+            ke += 0.5 * B[q].m *
+                  (B[q].v0[0] * B[q].v0[0] + B[q].v0[1] * B[q].v0[1] +
+                   B[q].v0[2] * B[q].v0[2]);
+        }
+        double w0 = wtime();
+        int irank = 0, istart = 1, nn = n;
+        gpupot_(&irank, &istart, &nn, &nn, m.data(), (double(*)[3])x.data(),
+                ph.data());
+        abi_check();
+        t_pot += wtime() - w0;
+        double pe = 0.0;
+        for (int q = 0; q < n; ++q) pe -= 0.5 * m[q] * ph[q];
+        *ekin = ke;
+        *epot = pe;
+        return ke + pe;
+    }
+
+    // One block step to the earliest due time.
+    void block() {
+        double tn = 1e300;
+        for (int q = 0; q < n; ++q) tn = std::min(tn, B[q].t0 + B[q].dt);
+        std::vector<int> A, R;
+        for (int q = 0; q < n; ++q)
+            if (B[q].t0 + B[q].dt == tn) {
+                A.push_back(q);
+                if (B[q].t0r + B[q].dtr == tn) R.push_back(q);
+            }
+        blocks++;
+        const int na = (int)A.size();
+        // Predicted position of every active body at tn.
+        std::vector<double> XP(3 * (size_t)na), VP(3 * (size_t)na);
+        for (int k = 0; k < na; ++k)
+            predict_cubic(B[A[k]], tn, &XP[3 * (size_t)k], &VP[3 * (size_t)k]);
+        // Irregular force with the current lists.
+        std::vector<double> FI(3 * (size_t)na), FID(3 * (size_t)na);
+        double w0 = wtime();
+#pragma omp parallel for schedule(dynamic, 16)
+        for (int k = 0; k < na; ++k)
+            irr_force(B, B[A[k]].list, tn, &XP[3 * (size_t)k], &VP[3 * (size_t)k],
+                      &FI[3 * (size_t)k], &FID[3 * (size_t)k]);
+        t_irr += wtime() - w0;
+        for (int k = 0; k < na; ++k) irr_pairs += (long long)B[A[k]].list.size();
+        irr_steps += na;
+
+        // Regular step for the bodies in R (engine call).
+        std::vector<int> rpos(n, -1);
+        for (int k = 0; k < na; ++k) rpos[A[k]] = k;
+        std::vector<double> FRN, FRDN;
+        std::vector<std::vector<int>> LN;
+        if (!R.empty()) {
+            send_j(tn);
+            std::vector<double> X(3 * R.size()), V(3 * R.size());
+            for (size_t k = 0; k < R.size(); ++k) {
+                const int a = rpos[R[k]];
+                for (int c = 0; c < 3; ++c) {
+                    X[3 * k + c] = XP[3 * (size_t)a + c];
+                    V[3 * k + c] = VP[3 * (size_t)a + c];
+                }
+            }
+            regular(R, X, V, FRN, FRDN, LN);
+            reg_steps += (long long)R.size();
+        }
+        std::vector<int> rk(n, -1);
+        for (size_t k = 0; k < R.size(); ++k) rk[R[k]] = (int)k;
+        // Irregular force of the new lists at tn (read-only pass: the
+        // corrector below modifies bodies that are neighbours of others).
+        // Also the gain/loss corrections of the 2nd/3rd force derivatives:
+        // a body that leaves the list carries its pair derivatives from the
+        // irregular to the regular polynomial, one that enters the list the
+        // other way (regcor_gpu.F's neighbour-change corrections).
+        std::vector<double> FIL(3 * R.size()), FIDL(3 * R.size());
+        std::vector<double> DF2(3 * R.size(), 0.0), DF3(3 * R.size(), 0.0);
+        w0 = wtime();
+#pragma omp parallel for schedule(dynamic, 16)
+        for (int q = 0; q < (int)R.size(); ++q) {
+            const int a = rpos[R[q]];
+            const double *xi = &XP[3 * (size_t)a], *vi = &VP[3 * (size_t)a];
+            irr_force(B, LN[q], tn, xi, vi, &FIL[3 * (size_t)q],
+                      &FIDL[3 * (size_t)q]);
+            // Total acceleration and jerk of i at tn.
+            double ai[3], ji[3];
+            for (int c = 0; c < 3; ++c) {
+                ai[c] = FIL[3 * (size_t)q + c] + FRN[3 * (size_t)q + c];
+                ji[c] = FIDL[3 * (size_t)q + c] + FRDN[3 * (size_t)q + c];
+            }
+            const std::vector<int> &Lo = B[R[q]].list, &Ln = LN[q];
+            size_t u = 0, w = 0;
+            while (u < Lo.size() || w < Ln.size()) {
+                int jb, sign;
+                if (w == Ln.size() || (u < Lo.size() && Lo[u] < Ln[w])) {
+                    jb = Lo[u++];
+                    sign = +1;            // lost: now a regular pair
+                } else if (u == Lo.size() || Ln[w] < Lo[u]) {
+                    jb = Ln[w++];
+                    sign = -1;            // gained: now an irregular pair
+                } else {
+                    ++u;
+                    ++w;
+                    continue;
+                }
+                const Body &bj = B[jb];
+                const double sj = tn - bj.t0;
+                double xj[3], vj[3], r_[3], v_[3], a_[3], j_[3], f2[3], f3[3];
+                predict_cubic(bj, tn, xj, vj);
+                for (int c = 0; c < 3; ++c) {
+                    const double F2j = bj.fi2[c] + bj.fr2[c] + bj.fr3[c] * (bj.t0 - bj.t0r);
+                    r_[c] = xj[c] - xi[c];
+                    v_[c] = vj[c] - vi[c];
+                    a_[c] = bj.a[c] + sj * (bj.j[c] + 0.5 * sj * F2j) - ai[c];
+                    j_[c] = bj.j[c] + sj * F2j - ji[c];
+                }
+                pair_d23(bj.m, r_, v_, a_, j_, f2, f3);
+                for (int c = 0; c < 3; ++c) {
+                    DF2[3 * (size_t)q + c] += sign * f2[c];
+                    DF3[3 * (size_t)q + c] += sign * f3[c];
+                }
+            }
+        }
+        for (size_t q = 0; q < R.size(); ++q) irr_pairs += (long long)LN[q].size();
+
+
+#pragma omp parallel for schedule(dynamic, 16)
+        for (int k = 0; k < na; ++k) {
+            Body &b = B[A[k]];
+            const double h = tn - b.t0;
+            const double *fin = &FI[3 * (size_t)k];
+            const double *fidn = &FID[3 * (size_t)k];
+            double fi2[3], fi3[3];
+            hermite23(b.fi, b.fid, fin, fidn, h, fi2, fi3);
+            const int r = rk[A[k]];
+            double newfi[3], newfid[3];
+            for (int c = 0; c < 3; ++c) {
+                newfi[c] = fin[c];
+                newfid[c] = fidn[c];
+            }
+            if (r >= 0) {
+                // Regular force of the old list at tn, regular Hermite over
+                // the regular interval, then switch to the new list.
+                const double H = tn - b.t0r;
+                const double *fiL2 = &FIL[3 * (size_t)r];
+                const double *fidL2 = &FIDL[3 * (size_t)r];
+                double frL[3], frdL[3], fr2[3], fr3[3];
+                for (int c = 0; c < 3; ++c) {
+                    frL[c] = FRN[3 * (size_t)r + c] + fiL2[c] - fin[c];
+                    frdL[c] = FRDN[3 * (size_t)r + c] + fidL2[c] - fidn[c];
+                }
+                hermite23(b.fr, b.frd, frL, frdL, H, fr2, fr3);
+                for (int c = 0; c < 3; ++c) {
+                    b.fr2[c] = fr2[c];
+                    b.fr3[c] = fr3[c];
+                }
+                // Corrector below uses the regular derivatives at t0.
+                const double s0 = b.t0 - b.t0r;
+                double F2[3], F3[3], xp[3], vp[3];
+                for (int c = 0; c < 3; ++c) {
+                    F2[c] = fi2[c] + b.fr2[c] + b.fr3[c] * s0;
+                    F3[c] = fi3[c] + b.fr3[c];
+                    xp[c] = XP[3 * (size_t)k + c];
+                    vp[c] = VP[3 * (size_t)k + c];
+                }
+                const double h3 = h * h * h, h4 = h3 * h, h5 = h4 * h;
+                for (int c = 0; c < 3; ++c) {
+                    b.x0[c] = xp[c] + F2[c] * h4 / 24.0 + F3[c] * h5 / 120.0;
+                    b.v0[c] = vp[c] + F2[c] * h3 / 6.0 + F3[c] * h4 / 24.0;
+                }
+                for (int c = 0; c < 3; ++c) {
+                    // Regular derivatives at tn, moved to the new list.
+                    b.fr2[c] = fr2[c] + fr3[c] * H + DF2[3 * (size_t)r + c];
+                    b.fr3[c] = fr3[c] + DF3[3 * (size_t)r + c];
+                    b.fr[c] = FRN[3 * (size_t)r + c];
+                    b.frd[c] = FRDN[3 * (size_t)r + c];
+                    newfi[c] = fiL2[c];
+                    newfid[c] = fidL2[c];
+                }
+                b.list = LN[r];
+                b.t0r = tn;
+            } else {
+                const double s0 = b.t0 - b.t0r;
+                double F2[3], F3[3];
+                for (int c = 0; c < 3; ++c) {
+                    F2[c] = fi2[c] + b.fr2[c] + b.fr3[c] * s0;
+                    F3[c] = fi3[c] + b.fr3[c];
+                }
+                const double h3 = h * h * h, h4 = h3 * h, h5 = h4 * h;
+                for (int c = 0; c < 3; ++c) {
+                    b.x0[c] = XP[3 * (size_t)k + c] + F2[c] * h4 / 24.0 + F3[c] * h5 / 120.0;
+                    b.v0[c] = VP[3 * (size_t)k + c] + F2[c] * h3 / 6.0 + F3[c] * h4 / 24.0;
+                }
+            }
+            // State at tn (irregular derivatives moved to the new list).
+            const double sr = tn - b.t0r;
+            for (int c = 0; c < 3; ++c) {
+                const double d2 = (r >= 0) ? DF2[3 * (size_t)r + c] : 0.0;
+                const double d3 = (r >= 0) ? DF3[3 * (size_t)r + c] : 0.0;
+                b.fi[c] = newfi[c];
+                b.fid[c] = newfid[c];
+                b.fi2[c] = fi2[c] + fi3[c] * h - d2;
+                b.fi3[c] = fi3[c] - d3;
+                const double frt = b.fr[c] + sr * (b.frd[c] + sr * (0.5 * b.fr2[c] + sr * b.fr3[c] / 6.0));
+                const double frdt = b.frd[c] + sr * (b.fr2[c] + 0.5 * sr * b.fr3[c]);
+                b.a[c] = b.fi[c] + frt;
+                b.j[c] = b.fid[c] + frdt;
+            }
+            b.t0 = tn;
+            if (r >= 0) {
+                // Neighbour radius towards NNBOPT (volume scaling), then the
+                // new regular step.
+                const double nb = std::max<double>(1.0, (double)b.list.size());
+                double f = std::cbrt((double)o.nnbopt / nb);
+                f = std::min(std::max(f, 0.8), 1.25);
+                b.rs *= f;
+            }
+            const double dtn = new_dt(b, tn, h, std::min(b.dtr, o.dtadj));
+            if (r >= 0) b.dtr = new_dtr(b, tn, b.dtr, dtn);
+            // Irregular step never beyond the next regular time.
+            double d = dtn;
+            while (tn + d > b.t0r + b.dtr && d > o.dtmin) d *= 0.5;
+            b.dt = d;
+        }
+        t_irr += wtime() - w0;
+        jpset(A);
+        t = tn;
+    }
+};
+
+void parse_args(int argc, char **argv, Opts &o, std::string &in) {
+    if (argc < 3) {
+        fprintf(stderr,
+                "usage: %s in.bin t_end [--etai x] [--etar x] [--dtadj x] "
+                "[--nnbopt k] [--lmax k] [--nnbmax k] [--send] [--prec p] "
+                "[--rs0 x]\n",
+                argv[0]);
+        exit(1);
+    }
+    in = argv[1];
+    o.t_end = atof(argv[2]);
+    for (int a = 3; a < argc; ++a) {
+        std::string s = argv[a];
+        auto nxt = [&]() { return a + 1 < argc ? argv[++a] : (char *)"0"; };
+        if (s == "--etai") o.etai = atof(nxt());
+        else if (s == "--etar") o.etar = atof(nxt());
+        else if (s == "--dtadj") o.dtadj = atof(nxt());
+        else if (s == "--nnbopt") o.nnbopt = atoi(nxt());
+        else if (s == "--lmax") o.lmax = atoi(nxt());
+        else if (s == "--nnbmax") o.nnbmax = atoi(nxt());
+        else if (s == "--send") o.use_predict = 0;
+        else if (s == "--prec") o.prec = atoi(nxt());
+        else if (s == "--rs0") o.rs0 = atof(nxt());
+        else {
+            fprintf(stderr, "unknown option %s\n", s.c_str());
+            exit(1);
+        }
+    }
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+    Opts o;
+    std::string in;
+    parse_args(argc, argv, o, in);
+    FILE *f = fopen(in.c_str(), "rb");
+    if (!f) {
+        perror(in.c_str());
+        return 1;
+    }
+    int n = 0;
+    if (fread(&n, sizeof(int), 1, f) != 1 || n <= 0) {
+        fprintf(stderr, "bad input\n");
+        return 1;
+    }
+    std::vector<double> m(n), x(3 * (size_t)n), v(3 * (size_t)n);
+    if (fread(m.data(), sizeof(double), n, f) != (size_t)n ||
+        fread(x.data(), sizeof(double), 3 * (size_t)n, f) != 3 * (size_t)n ||
+        fread(v.data(), sizeof(double), 3 * (size_t)n, f) != 3 * (size_t)n) {
+        fprintf(stderr, "short input\n");
+        return 1;
+    }
+    fclose(f);
+    o.nnbmax = std::min(o.nnbmax, std::min(n / 2, o.lmax - 50));
+    if (o.rs0 <= 0.0) {
+        // RS0 such that a uniform sphere of the half-mass density holds
+        // NNBOPT bodies: rs0^3 = NNBOPT/N (in units of the half-mass radius
+        // ~0.8 of a virialised model).
+        o.rs0 = 0.8 * std::cbrt((double)o.nnbopt / (double)n);
+    }
+
+    int one = 0;
+    gpunb_set_abort_(&one);
+    if (o.prec) gpunb_set_precision(o.prec);
+    Run R;
+    R.o = o;
+    R.n = n;
+    int nbmax = n + 10, irank = 0;
+    gpunb_open_(&nbmax, &irank);
+    R.abi_check();
+    double w0 = wtime();
+    R.init(m, x, v);
+    double ek, ep;
+    const double e0 = R.energy(&ek, &ep);
+    printf("{\"t\": 0.0, \"E\": %.15e, \"EK\": %.15e, \"EP\": %.15e, \"dE_E\": 0.0}\n",
+           e0, ek, ep);
+    fflush(stdout);
+    double next = o.dtadj;
+    while (R.t < o.t_end) {
+        R.block();
+        if (R.t >= next) {
+            const double e = R.energy(&ek, &ep);
+            const double wall = wtime() - w0;
+            long long nnbs = 0;
+            for (int q = 0; q < n; ++q) nnbs += (long long)R.B[q].list.size();
+            printf("{\"t\": %.6f, \"E\": %.15e, \"EK\": %.15e, \"EP\": %.15e, "
+                   "\"dE_E\": %.6e, \"blocks\": %lld, \"irr_steps\": %lld, "
+                   "\"reg_steps\": %lld, \"reg_calls\": %lld, \"reg_pairs\": %lld, "
+                   "\"irr_pairs\": %lld, \"overflows\": %lld, \"nnb_mean\": %.2f, "
+                   "\"t_reg\": %.3f, \"t_irr\": %.3f, \"t_pred\": %.3f, "
+                   "\"t_pot\": %.3f, \"wall\": %.3f}\n",
+                   R.t, e, ek, ep, (e - e0) / std::fabs(e0), R.blocks,
+                   R.irr_steps, R.reg_steps, R.reg_calls, R.reg_pairs,
+                   R.irr_pairs, R.overflows, (double)nnbs / n, R.t_reg, R.t_irr,
+                   R.t_pred, R.t_pot, wall);
+            fflush(stdout);
+            next += o.dtadj;
+        }
+    }
+    gpunb_close_();
+    return 0;
+}

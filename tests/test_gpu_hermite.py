@@ -1,0 +1,40 @@
+"""A full Hermite / Ahmad-Cohen run through the engine (host driver
+host/hermite_ac.cpp calling gpunb_open_/jpset_/predict_/send_/regf_/gpupot_):
+energy conservation over one N-body time unit.
+
+ETAI = ETAR = 0.02 are the values of every sample deck the reference ships
+(samples/*.input, third line); the bar there is |dE/E| < 1e-5 per time unit.
+The looser ETAI = 0.05, ETAR = 0.1 of the benchmark configuration is run too
+and must stay below 2e-4.
+"""
+import pytest
+
+from nbody6ppgpu_amd import ics, integrate
+
+pytestmark = pytest.mark.gpu
+
+
+def summary(tag, rec):
+    de = [abs(r["dE_E"]) for r in rec]
+    last = rec[-1]
+    print(f"{tag}: |dE/E| {['%.1e' % d for d in de[1:]]}; blocks {last['blocks']} "
+          f"irr {last['irr_steps']} reg {last['reg_steps']} "
+          f"nnb {last['nnb_mean']:.1f} overflows {last['overflows']} wall {last['wall']:.1f}s")
+    return max(de)
+
+
+@pytest.mark.parametrize("send", [False, True])
+def test_plummer_2048_reference_deck_steps(send):
+    c = ics.plummer(2048, seed=401)
+    rec = integrate.run(c.m, c.x, c.v, t_end=1.0, etai=0.02, etar=0.02,
+                        send=send, timeout=600)
+    assert rec[-1]["t"] == pytest.approx(1.0)
+    assert summary(f"eta 0.02/0.02 send={send}", rec) < 1e-5
+
+
+def test_king_kroupa_16k_benchmark_steps():
+    c = ics.king(16384, 6.0, seed=402, m=ics.kroupa(16384, seed=403))
+    rec = integrate.run(c.m, c.x, c.v, t_end=1.0, etai=0.05, etar=0.1,
+                        timeout=600)
+    assert rec[-1]["t"] == pytest.approx(1.0)
+    assert summary("King 16k Kroupa eta 0.05/0.1", rec) < 2e-4
