@@ -102,6 +102,8 @@ def main():
                     help="skip the CPU baseline leg")
     ap.add_argument("--prec", type=int, default=0,
                     help="pair arithmetic: 32, 64 or 65 (0 = library default)")
+    ap.add_argument("--no-hermite", action="store_true",
+                    help="skip the Hermite/Ahmad-Cohen integration leg")
     args = ap.parse_args()
     if args.gpus != 1:
         sys.exit("bench.py runs on one GPU only: this build measures the "
@@ -193,6 +195,31 @@ def main():
         log("bench: CPU baseline (reference AVX path)")
         cpu = cpu_baseline(c, h2, stepr)
 
+    # |dE/E| over one N-body time unit: the host Hermite / Ahmad-Cohen driver
+    # (host/hermite_ac.cpp) integrates the same cluster through the engine's
+    # ABI (device predictor + gpunb_regf_ + gpupot_), with the benchmark
+    # configuration's steps (ETAI 0.05, ETAR 0.1) and with the reference's
+    # sample-deck steps (ETAI = ETAR = 0.02).
+    hermite = []
+    if not args.no_hermite:
+        from nbody6ppgpu_amd import integrate
+        for etai, etar in ((0.05, 0.1), (0.02, 0.02)):
+            log(f"bench: Hermite/AC integration, ETAI {etai} ETAR {etar}")
+            rec = integrate.run(c.m, c.x, c.v, t_end=1.0, etai=etai, etar=etar,
+                                prec=args.prec, timeout=900)
+            last = rec[-1]
+            hermite.append({
+                "etai": etai, "etar": etar, "t_end": last["t"],
+                "dE_E": last["dE_E"],
+                "max_abs_dE_E": max(abs(r["dE_E"]) for r in rec),
+                "dE_E_checks": [r["dE_E"] for r in rec[1:]],
+                "regular_pairs": last["reg_pairs"],
+                "regular_pairs_per_s": last["reg_pairs"] / max(last["t_reg"], 1e-9),
+                "irregular_steps": last["irr_steps"],
+                "regular_steps": last["reg_steps"],
+                "blocks": last["blocks"], "nnb_mean": last["nnb_mean"],
+                "wall_s": last["wall"]})
+
     traffic = load_traffic()
     out = {
         "metric": "regular-force pair interactions/s at N=128k, % FP32 peak; "
@@ -234,6 +261,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "abi_ni1024_pairs_per_s": abi_rate,
+        "hermite_1tnb": hermite,
     }
     print(json.dumps(out), flush=True)
 

@@ -35,6 +35,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -174,6 +175,8 @@ struct Run {
     long long reg_pairs = 0, irr_pairs = 0, reg_calls = 0, irr_steps = 0,
               reg_steps = 0, blocks = 0, overflows = 0;
     double t_reg = 0.0, t_irr = 0.0, t_pred = 0.0, t_pot = 0.0;
+    // Block-step schedule: next time -> bodies due then (times are dyadic).
+    std::map<double, std::vector<int>> due;
     // staging for the ABI
     std::vector<double> h2, dtrv, xi, vi, acc, jrk, pot, jm, jx, jv;
     std::vector<int> lst;
@@ -384,6 +387,7 @@ struct Run {
         t_irr += wtime() - w0;
         irr_pairs += 0;
         for (int q = 0; q < n; ++q) irr_pairs += (long long)B[q].list.size();
+        for (int q = 0; q < n; ++q) due[B[q].t0 + B[q].dt].push_back(q);
         jpset(all);
     }
 
@@ -413,14 +417,15 @@ struct Run {
 
     // One block step to the earliest due time.
     void block() {
-        double tn = 1e300;
-        for (int q = 0; q < n; ++q) tn = std::min(tn, B[q].t0 + B[q].dt);
-        std::vector<int> A, R;
-        for (int q = 0; q < n; ++q)
-            if (B[q].t0 + B[q].dt == tn) {
-                A.push_back(q);
-                if (B[q].t0r + B[q].dtr == tn) R.push_back(q);
-            }
+        auto it = due.begin();
+        const double tn = it->first;
+        std::vector<int> A;
+        A.swap(it->second);
+        due.erase(it);
+        std::sort(A.begin(), A.end());
+        std::vector<int> R;
+        for (int q : A)
+            if (B[q].t0r + B[q].dtr == tn) R.push_back(q);
         blocks++;
         const int na = (int)A.size();
         // Predicted position of every active body at tn.
@@ -614,6 +619,7 @@ struct Run {
             b.dt = d;
         }
         t_irr += wtime() - w0;
+        for (int q : A) due[B[q].t0 + B[q].dt].push_back(q);
         jpset(A);
         t = tn;
     }
