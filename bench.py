@@ -104,6 +104,8 @@ def main():
                     help="pair arithmetic: 32, 64 or 65 (0 = library default)")
     ap.add_argument("--no-hermite", action="store_true",
                     help="skip the Hermite/Ahmad-Cohen integration leg")
+    ap.add_argument("--no-ensemble", action="store_true",
+                    help="skip the ensemble (64 clusters) leg")
     args = ap.parse_args()
     if args.gpus != 1:
         sys.exit("bench.py runs on one GPU only: this build measures the "
@@ -220,6 +222,40 @@ def main():
                 "blocks": last["blocks"], "nnb_mean": last["nnb_mean"],
                 "wall_s": last["wall"]})
 
+    # Ensemble: 64 independent Plummer clusters, N log-uniform in
+    # [1024, 65536], one regular block step of every body of every cluster,
+    # sequential per-cluster launches vs one batched launch vs its hipGraph,
+    # after the autotuner has chosen the launch shape.
+    ensemble = None
+    if not args.no_ensemble:
+        from nbody6ppgpu_amd import ensemble as ens
+        log("bench: ensemble workload (64 clusters)")
+        cl = ens.make_ensemble()
+        nnb_e = 300
+        gpunb.gpunb_ens_open([c[0].n for c in cl])
+        gpunb.gpunb_ens_send([(c[0].m, c[0].x, c[0].v) for c in cl])
+        eb = ens.upload(cl, nnb_e)
+        pairs = float(sum(c[0].n ** 2 for c in cl))
+        t_seq = ens.time_step(eb, nnb_e, "sequential", reps=3)
+        t_bat = ens.time_step(eb, nnb_e, "batched", reps=5)
+        t_gra = ens.time_step(eb, nnb_e, "graph", reps=5)
+        group, rounds, tiles, t_tuned = ens.autotune(eb, nnb_e, reps=3, log=log)
+        t_final = ens.time_step(eb, nnb_e, "graph", reps=10)
+        gpunb.gpunb_ens_close()
+        gpunb.gpunb_set_tuning(0, 0.0)
+        ensemble = {
+            "workload": "64 Plummer clusters, N log-uniform in [1024, 65536], "
+                        "equal-mass and Kroupa alternating, all bodies active",
+            "clusters": len(cl), "bodies": int(sum(c[0].n for c in cl)),
+            "pairs_per_step": pairs,
+            "sequential_pairs_per_s": pairs / t_seq,
+            "batched_pairs_per_s": pairs / t_bat,
+            "graph_pairs_per_s": pairs / t_gra,
+            "tuned_graph_pairs_per_s": pairs / t_final,
+            "tuned": {"group": group, "rounds": rounds,
+                      "bucket_tiles": {f"2^{b}": t for b, t in enumerate(tiles) if t}},
+        }
+
     traffic = load_traffic()
     out = {
         "metric": "regular-force pair interactions/s at N=128k, % FP32 peak; "
@@ -262,6 +298,7 @@ def main():
         "cpu_baseline": cpu,
         "abi_ni1024_pairs_per_s": abi_rate,
         "hermite_1tnb": hermite,
+        "ensemble": ensemble,
     }
     print(json.dumps(out), flush=True)
 

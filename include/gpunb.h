@@ -160,6 +160,44 @@ int gpunb_regf_dev(int ni, const double *d_h2, const double *d_dtr,
                    int *d_nnb, int *d_nnid, int *d_list, int *d_off,
                    int *d_total, long long list_cap);
 
+/* ------------------------------------------------------------------------ */
+/* Ensemble of independent clusters (no reference counterpart: the reference */
+/* runs one cluster per process, -D ENSEMBLE, nbody6.F:35).  All clusters'   */
+/* j sets stay in HBM and one launch computes every cluster's regular force; */
+/* indices in lists are local to the cluster.  Asynchronous, device arrays.  */
+/* ------------------------------------------------------------------------ */
+
+/* Open for ncl clusters of at most nbmax[c] j-particles each. */
+int gpunb_ens_open(int ncl, const int *nbmax);
+int gpunb_ens_close(void);
+
+/* Send all clusters' j-particles: host arrays concatenated in cluster order
+ * (nj[c] bodies of cluster c; m[], x[][3], v[][3]). */
+int gpunb_ens_send(int ncl, const int *nj, const double *m, const double *x,
+                   const double *v);
+
+/* Regular force of ni[c] i-particles of every cluster c against that cluster's
+ * j set, one launch pair for all.  i data and outputs are device arrays
+ * concatenated in cluster order (layouts as gpunb_regf_dev; d_off indexes
+ * the shared compact list, d_total receives its length). */
+int gpunb_ens_regf_dev(int ncl, const int *ni, const double *d_h2,
+                       const double *d_dtr, const double *d_xi,
+                       const double *d_vi, double *d_acc, double *d_jrk,
+                       double *d_pot, int nnbmax, int m_flag, int *d_nnb,
+                       int *d_nnid, int *d_list, int *d_off, int *d_total,
+                       long long list_cap);
+
+/* enable != 0: capture the ensemble launch sequence into a hipGraph and
+ * replay it while sizes and pointers stay the same (re-captured otherwise). */
+int gpunb_ens_graph(int enable);
+
+/* Autotuner output: j tiles (64 j) per slice for clusters whose j count lies
+ * in [2^b, 2^(b+1)), b < nbucket; 0 = automatic.  Returns 0 or -1. */
+int gpunb_ens_set_bucket_tiles(int nbucket, const int *tiles);
+
+/* Pair interactions of the last ensemble launch (sum over clusters ni*nj). */
+long long gpunb_ens_pairs(void);
+
 /* Wait for all work on the library stream; returns 0 or -1 (error set). */
 int gpunb_sync(void);
 

@@ -99,8 +99,10 @@ struct State {
     int ndev = 0;
     int rank = 0;
     hipStream_t st = nullptr;
-    hipEvent_t ev_send = nullptr, ev_jp = nullptr;
-    bool ev_send_live = false, ev_jp_live = false;
+    hipEvent_t ev_send = nullptr, ev_jp = nullptr, ev_desc = nullptr;
+    bool ev_send_live = false, ev_jp_live = false, ev_desc_live = false;
+    HostBuf<ClusterDesc> h_desc;
+    DevBuf<ClusterDesc> d_desc;
 
     int nbmax = 0;
     int nj = 0;       // current j count used by regf
@@ -204,6 +206,7 @@ void devinit(int irank) {
     HIPCHK(hipStreamCreateWithFlags(&S.st, hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&S.ev_send, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&S.ev_jp, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&S.ev_desc, hipEventDisableTiming));
     S.jst.ensure(1);
     S.err.ensure(1);
     S.h_err.ensure(1);
@@ -319,6 +322,10 @@ void ensure_regf_scratch(int ni_c, const Decomp &d) {
 }
 
 // Enqueue the regular-force kernels for device-resident i data.
+void wait_event(hipEvent_t ev, bool &live);
+
+// Enqueue the regular-force kernels for device-resident i data: one launch
+// pair per chunk of kChunk i, each a one-cluster table.
 void enqueue_regf(int ni, const double *xi, const double *vi, const double *h2,
                   const double *dtr, double *acc, double *jrk, double *pot,
                   int nnbmax, int m_flag, int *nnb, int *nnid, int *list,
@@ -326,32 +333,50 @@ void enqueue_regf(int ni, const double *xi, const double *vi, const double *h2,
     const int group = pick_group();
     const int prec = pick_prec();
     const int ntile = cdiv(S.nj, kTile);
-    for (int c0 = 0; c0 < ni; c0 += kChunk) {
+    const int nch = cdiv(ni, kChunk);
+    wait_event(S.ev_desc, S.ev_desc_live);
+    S.h_desc.ensure((size_t)nch);
+    S.d_desc.ensure((size_t)nch);
+    for (int c = 0; c < nch; ++c) {
+        const int c0 = c * kChunk;
         const int nc = std::min(kChunk, ni - c0);
         Decomp d = decompose(nc, ntile, nnbmax, m_flag, group, prec);
         ensure_regf_scratch(nc, d);
+        ClusterDesc &D = S.h_desc.p[c];
+        memset(&D, 0, sizeof D);
+        D.nj = S.nj;
+        D.ntile = ntile;
+        D.i0 = c0;
+        D.ni = nc;
+        D.n_iw = cdiv(nc, 64);
+        D.nslice = ntile == 0 ? 1 : d.nslice;
+        D.cap = ntile == 0 ? 1 : d.cap;
+        D.ipb = pass2_ipb(nc);
+    }
+    HIPCHK(hipMemcpyAsync(S.d_desc.p, S.h_desc.p, nch * sizeof(ClusterDesc),
+                          hipMemcpyHostToDevice, S.st));
+    HIPCHK(hipEventRecord(S.ev_desc, S.st));
+    S.ev_desc_live = true;
+    for (int c = 0; c < nch; ++c) {
+        const ClusterDesc &D = S.h_desc.p[c];
         RegfLaunch L;
+        memset(&L, 0, sizeof L);
         L.jp = S.jp.p;
-        L.nj = S.nj;
-        L.ntile = ntile;
         L.jst = S.jst.p;
-        L.xi = xi + 3 * (size_t)c0;
-        L.vi = vi + 3 * (size_t)c0;
-        L.h2 = h2 + c0;
-        L.dtr = dtr + c0;
-        L.ni = nc;
-        L.nslice = d.nslice;
-        L.cap = d.cap;
+        L.xi = xi;
+        L.vi = vi;
+        L.h2 = h2;
+        L.dtr = dtr;
         L.part = S.part.p;
         L.cnt = S.cnt.p;
         L.seg = S.seg.p;
-        L.acc = acc + 3 * (size_t)c0;
-        L.jrk = jrk + 3 * (size_t)c0;
-        L.pot = pot + c0;
-        L.nnb = nnb + c0;
-        L.nnid = nnid ? nnid + c0 : nullptr;
+        L.acc = acc;
+        L.jrk = jrk;
+        L.pot = pot;
+        L.nnb = nnb;
+        L.nnid = nnid;
         L.list = list;
-        L.off = off + c0;
+        L.off = off;
         L.total = total;
         L.list_cap = list_cap;
         L.nnbmax = nnbmax;
@@ -359,11 +384,10 @@ void enqueue_regf(int ni, const double *xi, const double *vi, const double *h2,
         L.group = group;
         L.prec = prec;
         L.err = S.err.p;
-        if (ntile == 0) {
-            // No j-particles: one empty slice, zero forces, empty lists.
-            L.nslice = 1;
-            L.cap = 1;
-        }
+        L.desc = S.d_desc.p + c;
+        L.ncl = 1;
+        L.ntask = (long long)D.n_iw * D.nslice;
+        L.nblk2 = cdiv(D.ni, D.ipb);
         int e0 = -1, e1 = -1;
         if (S.timing) {
             e0 = timing_event();
@@ -374,7 +398,7 @@ void enqueue_regf(int ni, const double *xi, const double *vi, const double *h2,
             e1 = timing_event();
             HIPCHK(hipEventRecord(S.ev_pool[e1], S.st));
             S.ev_pairs.emplace_back(e0, e1);
-            S.ev_pairs_work.push_back((long long)nc * S.nj);
+            S.ev_pairs_work.push_back((long long)D.ni * S.nj);
         }
         launch_regf_pass2(S.st, L);
         HIPCHK(hipGetLastError());
@@ -442,7 +466,7 @@ void do_close() {
     }
     set_device();
     HIPCHK(hipStreamSynchronize(S.st));
-    S.ev_send_live = S.ev_jp_live = false;
+    S.ev_send_live = S.ev_jp_live = S.ev_desc_live = false;
     S.jp.release();
     S.d_send.release();
     S.h_send.release();
@@ -723,6 +747,258 @@ void do_jpget(int n, double (*x)[3], double (*v)[3]) {
     memcpy(v, S.h_pout.p + 3 * (size_t)n, 3 * (size_t)n * sizeof(double));
 }
 
+// ---------------------------------------------------------------------------
+// Ensemble of independent clusters: every cluster's j set lives in one JRec
+// array (each padded to whole tiles), and one launch pair computes the
+// regular force of all clusters, each i against its own cluster only.  The
+// launch sequence can be captured into a hipGraph and replayed while the
+// sizes stay the same.
+// ---------------------------------------------------------------------------
+
+struct Ens {
+    bool open = false;
+    int ncl = 0;
+    std::vector<int> capj, j0, nj;
+    int jcap = 0;
+    DevBuf<JRec> jp;
+    DevBuf<JStats> jst;
+    HostBuf<double> h_send;
+    DevBuf<double> d_send;
+    HostBuf<ClusterDesc> h_desc;
+    DevBuf<ClusterDesc> d_desc;
+    DevBuf<double> part;
+    DevBuf<int> cnt;
+    DevBuf<uint16_t> seg;
+    std::vector<int> bucket_tiles = std::vector<int>(24, 0);
+    bool graph_on = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<long long> gkey;
+    long long last_pairs = 0;
+};
+Ens E;
+
+void ens_graph_drop() {
+    if (E.gexec) (void)hipGraphExecDestroy(E.gexec);
+    if (E.graph) (void)hipGraphDestroy(E.graph);
+    E.gexec = nullptr;
+    E.graph = nullptr;
+    E.gkey.clear();
+}
+
+void ens_close() {
+    if (!E.open) return;
+    set_device();
+    HIPCHK(hipStreamSynchronize(S.st));
+    ens_graph_drop();
+    E.jp.release();
+    E.jst.release();
+    E.h_send.release();
+    E.d_send.release();
+    E.h_desc.release();
+    E.d_desc.release();
+    E.part.release();
+    E.cnt.release();
+    E.seg.release();
+    E.open = false;
+    E.ncl = 0;
+}
+
+void ens_open(int ncl, const int *nbmax) {
+    devinit(0);
+    REQUIRE(ncl > 0, "gpunb_ens_open: ncl must be > 0");
+    if (E.open) ens_close();
+    set_device();
+    E.ncl = ncl;
+    E.capj.assign(nbmax, nbmax + ncl);
+    E.j0.resize(ncl);
+    E.nj.assign(ncl, 0);
+    long long off = 0;
+    for (int c = 0; c < ncl; ++c) {
+        REQUIRE(nbmax[c] >= 0, "gpunb_ens_open: nbmax must be >= 0");
+        E.j0[c] = (int)off;
+        off += round_up(std::max(nbmax[c], 1), kTile);
+    }
+    REQUIRE(off + kTile < (1LL << 31), "gpunb_ens_open: ensemble too large");
+    E.jcap = (int)off + kTile;
+    E.jp.ensure((size_t)E.jcap);
+    E.jst.ensure(1);
+    launch_stats_reset(S.st, E.jst.p, nullptr);
+    launch_pack_j(S.st, nullptr, nullptr, nullptr, 0, E.jcap, E.jp.p, E.jst.p,
+                  S.err.p);
+    HIPCHK(hipStreamSynchronize(S.st));
+    E.open = true;
+}
+
+void ens_send(int ncl, const int *nj, const double *m, const double (*x)[3],
+              const double (*v)[3]) {
+    REQUIRE(E.open, "gpunb_ens_send: ensemble is not open");
+    REQUIRE(ncl == E.ncl, "gpunb_ens_send: ncl differs from gpunb_ens_open");
+    set_device();
+    long long tot = 0;
+    for (int c = 0; c < ncl; ++c) {
+        REQUIRE(nj[c] >= 0 && nj[c] <= E.capj[c],
+                "gpunb_ens_send: nj outside 0..nbmax of the cluster");
+        tot += nj[c];
+    }
+    HIPCHK(hipStreamSynchronize(S.st));   // staging and graphs may be in use
+    E.h_send.ensure(std::max<size_t>(7 * (size_t)tot, 7));
+    E.d_send.ensure(std::max<size_t>(7 * (size_t)tot, 7));
+    memcpy(E.h_send.p, m, tot * sizeof(double));
+    memcpy(E.h_send.p + tot, x, 3 * tot * sizeof(double));
+    memcpy(E.h_send.p + 4 * tot, v, 3 * tot * sizeof(double));
+    HIPCHK(hipMemcpyAsync(E.d_send.p, E.h_send.p, 7 * tot * sizeof(double),
+                          hipMemcpyHostToDevice, S.st));
+    launch_stats_reset(S.st, E.jst.p, nullptr);
+    long long o = 0;
+    for (int c = 0; c < ncl; ++c) {
+        const int pad = std::max(round_up(nj[c], kTile), round_up(E.nj[c], kTile));
+        launch_pack_j(S.st, E.d_send.p + o, E.d_send.p + tot + 3 * o,
+                      E.d_send.p + 4 * tot + 3 * o, nj[c], pad,
+                      E.jp.p + E.j0[c], E.jst.p, S.err.p);
+        E.nj[c] = nj[c];
+        o += nj[c];
+    }
+    HIPCHK(hipGetLastError());
+}
+
+// Tiles per j slice shared by every cluster so that the launch holds about
+// `rounds` times the resident-wave capacity; a per-size-bucket table
+// (gpunb_ens_set_bucket_tiles, the autotuner's output) overrides it.
+int ens_slice_tiles(const int *ni, int group, int prec, int m_flag) {
+    const double target = resident_waves(m_flag, group, prec) * pick_rounds();
+    int lo = 1, hi = 1024;
+    auto waves = [&](int L) {
+        double w = 0;
+        for (int c = 0; c < E.ncl; ++c)
+            w += (double)cdiv(ni[c], 64) * std::max(1, cdiv(cdiv(E.nj[c], kTile), L));
+        return w;
+    };
+    while (lo < hi) {               // smallest L with waves(L) <= target
+        const int mid = (lo + hi) / 2;
+        if (waves(mid) <= target) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo;
+}
+
+void ens_regf_dev(int ncl, const int *ni, const double *h2, const double *dtr,
+                  const double *xi, const double *vi, double *acc, double *jrk,
+                  double *pot, int nnbmax, int m_flag, int *nnb, int *nnid,
+                  int *list, int *off, int *total, long long list_cap) {
+    REQUIRE(E.open, "gpunb_ens_regf_dev: ensemble is not open");
+    REQUIRE(ncl == E.ncl, "gpunb_ens_regf_dev: ncl differs from gpunb_ens_open");
+    REQUIRE(nnbmax >= 0, "gpunb_ens_regf_dev: nnbmax must be >= 0");
+    set_device();
+    const int group = pick_group();
+    const int prec = pick_prec();
+    const int L = ens_slice_tiles(ni, group, prec, m_flag);
+    // The launch key: a graph is replayed only for identical launches.
+    std::vector<long long> key = {ncl, nnbmax, m_flag, group, prec, L,
+                                  (long long)h2, (long long)dtr,
+                                  (long long)xi, (long long)vi,
+                                  (long long)acc, (long long)jrk,
+                                  (long long)pot, (long long)nnb,
+                                  (long long)nnid, (long long)list,
+                                  (long long)off, (long long)total, list_cap};
+    for (int c = 0; c < ncl; ++c) {
+        key.push_back(ni[c]);
+        key.push_back(E.nj[c]);
+        key.push_back(E.bucket_tiles[std::min(23, std::max(0, (int)std::log2(std::max(E.nj[c], 1))))]);
+    }
+    if (E.graph_on && E.gexec && key == E.gkey) {
+        HIPCHK(hipGraphLaunch(E.gexec, S.st));
+        return;
+    }
+    // Cluster table.
+    HIPCHK(hipStreamSynchronize(S.st));   // the table may be in use
+    E.h_desc.ensure((size_t)ncl);
+    E.d_desc.ensure((size_t)ncl);
+    long long task = 0, pairs = 0, segel = 0, blk = 0, i0 = 0, work = 0;
+    for (int c = 0; c < ncl; ++c) {
+        REQUIRE(ni[c] >= 0, "gpunb_ens_regf_dev: ni must be >= 0");
+        ClusterDesc &D = E.h_desc.p[c];
+        memset(&D, 0, sizeof D);
+        const int ntile = cdiv(E.nj[c], kTile);
+        const int b = std::min(23, std::max(0, (int)std::log2(std::max(E.nj[c], 1))));
+        const int Lc = E.bucket_tiles[b] > 0 ? E.bucket_tiles[b] : L;
+        D.j0 = E.j0[c];
+        D.nj = E.nj[c];
+        D.ntile = ntile;
+        D.i0 = (int)i0;
+        D.ni = ni[c];
+        D.n_iw = cdiv(ni[c], 64);
+        D.nslice = std::max(1, std::max(cdiv(ntile, Lc), cdiv(ntile, 1024)));
+        D.nslice = std::min(D.nslice, std::max(ntile, 1));
+        D.cap = std::max(1, std::min(cdiv(ntile, D.nslice) * kTile, nnbmax + 1));
+        D.ipb = pass2_ipb(ni[c]);
+        D.task0 = task;
+        D.pbase = pairs;
+        D.sbase = segel;
+        D.blk0 = blk;
+        task += (long long)D.n_iw * D.nslice;
+        const long long pr = (long long)round_up(std::max(ni[c], 1), 64) * D.nslice;
+        pairs += pr;
+        segel += pr * D.cap;
+        blk += cdiv(ni[c], D.ipb);
+        i0 += ni[c];
+        work += (long long)ni[c] * E.nj[c];
+    }
+    E.last_pairs = work;
+    E.part.ensure((size_t)pairs * 8);
+    E.cnt.ensure((size_t)pairs);
+    E.seg.ensure((size_t)std::max<long long>(segel, 1));
+    HIPCHK(hipMemcpyAsync(E.d_desc.p, E.h_desc.p, ncl * sizeof(ClusterDesc),
+                          hipMemcpyHostToDevice, S.st));
+    HIPCHK(hipStreamSynchronize(S.st));
+    RegfLaunch R;
+    memset(&R, 0, sizeof R);
+    R.jp = E.jp.p;
+    R.jst = E.jst.p;
+    R.xi = xi;
+    R.vi = vi;
+    R.h2 = h2;
+    R.dtr = dtr;
+    R.part = E.part.p;
+    R.cnt = E.cnt.p;
+    R.seg = E.seg.p;
+    R.acc = acc;
+    R.jrk = jrk;
+    R.pot = pot;
+    R.nnb = nnb;
+    R.nnid = nnid;
+    R.list = list;
+    R.off = off;
+    R.total = total;
+    R.list_cap = list_cap;
+    R.nnbmax = nnbmax;
+    R.m_flag = m_flag;
+    R.group = group;
+    R.prec = prec;
+    R.err = S.err.p;
+    R.desc = E.d_desc.p;
+    R.ncl = ncl;
+    R.ntask = task;
+    R.nblk2 = blk;
+    auto enqueue = [&]() {
+        launch_zero_int(S.st, total, 1);
+        launch_regf_pass1(S.st, R);
+        launch_regf_pass2(S.st, R);
+    };
+    if (!E.graph_on) {
+        enqueue();
+        HIPCHK(hipGetLastError());
+        return;
+    }
+    ens_graph_drop();
+    HIPCHK(hipStreamBeginCapture(S.st, hipStreamCaptureModeThreadLocal));
+    enqueue();
+    HIPCHK(hipStreamEndCapture(S.st, &E.graph));
+    HIPCHK(hipGraphInstantiate(&E.gexec, E.graph, nullptr, nullptr, 0));
+    E.gkey = key;
+    HIPCHK(hipGraphLaunch(E.gexec, S.st));
+}
+
 }  // namespace
 }  // namespace gpunb
 
@@ -880,6 +1156,58 @@ GPUNB_EXPORT int gpunb_timing_read(double *ms_total, long long *launches,
         S.ev_used = 0;
     });
 }
+
+GPUNB_EXPORT int gpunb_ens_open(int ncl, const int *nbmax) {
+    return guarded("gpunb_ens_open", [&] { ens_open(ncl, nbmax); });
+}
+
+GPUNB_EXPORT int gpunb_ens_close(void) {
+    return guarded("gpunb_ens_close", [&] { ens_close(); });
+}
+
+GPUNB_EXPORT int gpunb_ens_send(int ncl, const int *nj, const double *m,
+                                const double *x, const double *v) {
+    return guarded("gpunb_ens_send", [&] {
+        ens_send(ncl, nj, m, (const double(*)[3])x, (const double(*)[3])v);
+    });
+}
+
+GPUNB_EXPORT int gpunb_ens_regf_dev(int ncl, const int *ni, const double *d_h2,
+                                    const double *d_dtr, const double *d_xi,
+                                    const double *d_vi, double *d_acc,
+                                    double *d_jrk, double *d_pot, int nnbmax,
+                                    int m_flag, int *d_nnb, int *d_nnid,
+                                    int *d_list, int *d_off, int *d_total,
+                                    long long list_cap) {
+    return guarded("gpunb_ens_regf_dev", [&] {
+        ens_regf_dev(ncl, ni, d_h2, d_dtr, d_xi, d_vi, d_acc, d_jrk, d_pot,
+                     nnbmax, m_flag, d_nnb, d_nnid, d_list, d_off, d_total,
+                     list_cap);
+    });
+}
+
+GPUNB_EXPORT int gpunb_ens_graph(int enable) {
+    return guarded("gpunb_ens_graph", [&] {
+        if (S.devinit) {
+            set_device();
+            HIPCHK(hipStreamSynchronize(S.st));
+        }
+        ens_graph_drop();
+        E.graph_on = enable != 0;
+    });
+}
+
+GPUNB_EXPORT int gpunb_ens_set_bucket_tiles(int nbucket, const int *tiles) {
+    return guarded("gpunb_ens_set_bucket_tiles", [&] {
+        REQUIRE(nbucket >= 0 && nbucket <= 24,
+                "gpunb_ens_set_bucket_tiles: at most 24 buckets");
+        for (int b = 0; b < 24; ++b)
+            E.bucket_tiles[b] = (b < nbucket && tiles) ? std::max(0, tiles[b]) : 0;
+        ens_graph_drop();
+    });
+}
+
+GPUNB_EXPORT long long gpunb_ens_pairs(void) { return E.last_pairs; }
 
 GPUNB_EXPORT int gpunb_set_tuning(int group, double rounds) {
     return guarded("gpunb_set_tuning", [&] {

@@ -51,6 +51,27 @@ constexpr double kPadPos = 1.0e18;
 // Tile of j-particles between FP32 -> FP64 folds of the partial sums.
 constexpr int kTile = 64;
 
+// One independent cluster of a regular-force launch.  A launch covers ncl
+// clusters (the drop-in path uses one per i chunk, the ensemble path one per
+// star cluster); pass-1 tasks and pass-2 workgroups of all clusters form one
+// grid, each block finding its cluster by binary search over task0 / blk0.
+struct ClusterDesc {
+    long long task0;   // first pass-1 task (wavefront) of the cluster
+    long long pbase;   // first (i, slice) pair of the cluster in part / cnt
+    long long sbase;   // first element of the cluster's segments in seg
+    long long blk0;    // first pass-2 workgroup of the cluster
+    int j0;            // first JRec of the cluster (multiple of kTile)
+    int nj;            // real j count
+    int ntile;         // j tiles (kTile j each, padded)
+    int i0;            // first i of the cluster in the i / output arrays
+    int ni;
+    int n_iw;          // i wavefronts: ceil(ni / 64)
+    int nslice;        // j slices per i wavefront
+    int cap;           // neighbour segment capacity per (i, slice)
+    int ipb;           // pass-2 i per workgroup
+    int pad;
+};
+
 struct RegfLaunch {
     // j side
     const JRec *jp;
@@ -76,6 +97,11 @@ struct RegfLaunch {
     int group;       // pairs per branch-free group of pass 1 (2 or 4)
     int prec;        // 32: FP32 pair arithmetic; 64/65: FP64 (+ Newton steps)
     unsigned *err;
+    // cluster table (device) and grid sizes
+    const ClusterDesc *desc;
+    int ncl;
+    long long ntask;   // pass-1 tasks over all clusters
+    long long nblk2;   // pass-2 workgroups over all clusters
 };
 
 // Kernel launchers (kernels.hip).  All asynchronous on `st`.
@@ -93,6 +119,7 @@ void launch_jpset(hipStream_t st, int n, const int *addr, const double *src,
 void launch_jget(hipStream_t st, int n, const JRec *jp, double *x, double *v);
 void launch_regf_pass1(hipStream_t st, const RegfLaunch &L);
 void launch_regf_pass2(hipStream_t st, const RegfLaunch &L);
+int pass2_ipb(int ni);   // pass-2 i per workgroup for a cluster of ni
 void launch_zero_int(hipStream_t st, int *p, int n);
 void launch_pot(hipStream_t st, const PRec *p, int n, int ntile, int i0,
                 int ni, int nslice, double *part, double *pot);

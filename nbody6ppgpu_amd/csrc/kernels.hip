@@ -91,6 +91,28 @@ __device__ inline double speed_up(double vx, double vy, double vz) {
     return sqrt(sq3d(vx, vy, vz)) * (1.0 + 1e-15);
 }
 
+// Cluster of a pass-1 task / pass-2 workgroup: last c with key(c) <= k.
+__device__ inline int find_task(const ClusterDesc *__restrict__ d, int ncl,
+                                long long k) {
+    int lo = 0, hi = ncl - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (d[mid].task0 <= k) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+__device__ inline int find_blk(const ClusterDesc *__restrict__ d, int ncl,
+                               long long k) {
+    int lo = 0, hi = ncl - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (d[mid].blk0 <= k) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -239,19 +261,30 @@ __global__ void k_zero_int(int *p, int n) {
 
 template <bool MFLAG, int G>
 __global__ __launch_bounds__(64) void k_regf_pass1(
-    const JRec *__restrict__ jp, int nj, int ntile,
-    const JStats *__restrict__ jst, const double *__restrict__ xi,
-    const double *__restrict__ vi, const double *__restrict__ h2,
-    const double *__restrict__ dtr, int ni, int n_iw, int nslice, int cap,
-    double *__restrict__ part, int *__restrict__ cnt,
-    uint16_t *__restrict__ seg, unsigned *__restrict__ err) {
+    const JRec *__restrict__ jp_all, const ClusterDesc *__restrict__ desc,
+    int ncl, long long ntask, const JStats *__restrict__ jst,
+    const double *__restrict__ xi_all, const double *__restrict__ vi_all,
+    const double *__restrict__ h2_all, const double *__restrict__ dtr_all,
+    double *__restrict__ part_all, int *__restrict__ cnt_all,
+    uint16_t *__restrict__ seg_all, unsigned *__restrict__ err) {
     // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
     // contiguous range of tasks so that it streams only its share of the j
-    // array through its own L2.  Tasks are slice-major.
-    const int ntask = n_iw * nslice;
-    const int per = (ntask + 7) >> 3;
-    const int task = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
-    if (task >= ntask) return;
+    // array through its own L2.  Tasks are slice-major within a cluster.
+    const long long per = (ntask + 7) >> 3;
+    const long long gtask = (long long)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (gtask >= ntask) return;
+    const ClusterDesc &D = desc[find_task(desc, ncl, gtask)];
+    const int nj = D.nj, ntile = D.ntile, ni = D.ni, n_iw = D.n_iw;
+    const int nslice = D.nslice, cap = D.cap;
+    const JRec *__restrict__ jp = jp_all + D.j0;
+    const double *__restrict__ xi = xi_all + 3 * (size_t)D.i0;
+    const double *__restrict__ vi = vi_all + 3 * (size_t)D.i0;
+    const double *__restrict__ h2 = h2_all + D.i0;
+    const double *__restrict__ dtr = dtr_all + D.i0;
+    double *__restrict__ part = part_all + 8 * D.pbase;
+    int *__restrict__ cnt = cnt_all + D.pbase;
+    uint16_t *__restrict__ seg = seg_all + D.sbase;
+    const int task = (int)(gtask - D.task0);
     const int s = task / n_iw;
     const int iw = task - s * n_iw;
     const int lane = threadIdx.x;
@@ -430,16 +463,30 @@ __global__ __launch_bounds__(64) void k_regf_pass1(
 
 template <bool MFLAG, int G, int NR>
 __global__ __launch_bounds__(64) void k_regf_pass1_f64(
-    const JRec *__restrict__ jp, int nj, int ntile,
-    const JStats *__restrict__ jst, const double *__restrict__ xi,
-    const double *__restrict__ vi, const double *__restrict__ h2,
-    const double *__restrict__ dtr, int ni, int n_iw, int nslice, int cap,
-    double *__restrict__ part, int *__restrict__ cnt,
-    uint16_t *__restrict__ seg, unsigned *__restrict__ err) {
-    const int ntask = n_iw * nslice;
-    const int per = (ntask + 7) >> 3;
-    const int task = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
-    if (task >= ntask) return;
+    const JRec *__restrict__ jp_all, const ClusterDesc *__restrict__ desc,
+    int ncl, long long ntask, const JStats *__restrict__ jst,
+    const double *__restrict__ xi_all, const double *__restrict__ vi_all,
+    const double *__restrict__ h2_all, const double *__restrict__ dtr_all,
+    double *__restrict__ part_all, int *__restrict__ cnt_all,
+    uint16_t *__restrict__ seg_all, unsigned *__restrict__ err) {
+    // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
+    // contiguous range of tasks so that it streams only its share of the j
+    // array through its own L2.  Tasks are slice-major within a cluster.
+    const long long per = (ntask + 7) >> 3;
+    const long long gtask = (long long)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (gtask >= ntask) return;
+    const ClusterDesc &D = desc[find_task(desc, ncl, gtask)];
+    const int nj = D.nj, ntile = D.ntile, ni = D.ni, n_iw = D.n_iw;
+    const int nslice = D.nslice, cap = D.cap;
+    const JRec *__restrict__ jp = jp_all + D.j0;
+    const double *__restrict__ xi = xi_all + 3 * (size_t)D.i0;
+    const double *__restrict__ vi = vi_all + 3 * (size_t)D.i0;
+    const double *__restrict__ h2 = h2_all + D.i0;
+    const double *__restrict__ dtr = dtr_all + D.i0;
+    double *__restrict__ part = part_all + 8 * D.pbase;
+    int *__restrict__ cnt = cnt_all + D.pbase;
+    uint16_t *__restrict__ seg = seg_all + D.sbase;
+    const int task = (int)(gtask - D.task0);
     const int s = task / n_iw;
     const int iw = task - s * n_iw;
     const int lane = threadIdx.x;
@@ -604,19 +651,34 @@ __global__ __launch_bounds__(64) void k_regf_pass1_f64(
 constexpr int kP2I = 32;
 
 __global__ __launch_bounds__(256) void k_regf_pass2(
-    const JRec *__restrict__ jp, const double *__restrict__ xi, int ni,
-    int nslice, int cap, int ntile, const double *__restrict__ part,
-    const int *__restrict__ cnt, const uint16_t *__restrict__ seg,
-    double *__restrict__ acc, double *__restrict__ jrk,
-    double *__restrict__ pot, int *__restrict__ nnb_out,
-    int *__restrict__ nnid, int *__restrict__ list, int *__restrict__ off,
-    int *__restrict__ total, long long list_cap, int nnbmax, int ipb,
+    const JRec *__restrict__ jp_all, const ClusterDesc *__restrict__ desc,
+    int ncl, const double *__restrict__ xi_all,
+    const double *__restrict__ part_all, const int *__restrict__ cnt_all,
+    const uint16_t *__restrict__ seg_all, double *__restrict__ acc_all,
+    double *__restrict__ jrk_all, double *__restrict__ pot_all,
+    int *__restrict__ nnb_all, int *__restrict__ nnid_all,
+    int *__restrict__ list, int *__restrict__ off_all,
+    int *__restrict__ total, long long list_cap, int nnbmax,
     unsigned *__restrict__ err) {
     __shared__ int s_tot[kP2I];
     __shared__ int s_off[kP2I];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int i0 = blockIdx.x * ipb;
+    const ClusterDesc &D = desc[find_blk(desc, ncl, blockIdx.x)];
+    const int ni = D.ni, nslice = D.nslice, cap = D.cap, ntile = D.ntile;
+    const int ipb = D.ipb;
+    const int i0 = (int)(blockIdx.x - D.blk0) * ipb;
+    const JRec *__restrict__ jp = jp_all + D.j0;
+    const double *__restrict__ xi = xi_all + 3 * (size_t)D.i0;
+    const double *__restrict__ part = part_all + 8 * D.pbase;
+    const int *__restrict__ cnt = cnt_all + D.pbase;
+    const uint16_t *__restrict__ seg = seg_all + D.sbase;
+    double *__restrict__ acc = acc_all + 3 * (size_t)D.i0;
+    double *__restrict__ jrk = jrk_all + 3 * (size_t)D.i0;
+    double *__restrict__ pot = pot_all + D.i0;
+    int *__restrict__ nnb_out = nnb_all + D.i0;
+    int *__restrict__ nnid = nnid_all ? nnid_all + D.i0 : nullptr;
+    int *__restrict__ off = off_all + D.i0;
 
     for (int r = wave; r < ipb; r += 4) {
         const int i = i0 + r;
@@ -849,61 +911,59 @@ void launch_zero_int(hipStream_t st, int *p, int n) {
 }
 
 template <bool MFLAG, int G>
-static void launch_pass1(hipStream_t st, const RegfLaunch &L, int n_iw,
-                         int nblk) {
-    hipLaunchKernelGGL((k_regf_pass1<MFLAG, G>), dim3(nblk), dim3(64), 0, st,
-                       L.jp, L.nj, L.ntile, L.jst, L.xi, L.vi, L.h2, L.dtr,
-                       L.ni, n_iw, L.nslice, L.cap, L.part, L.cnt, L.seg,
-                       L.err);
+static void launch_pass1(hipStream_t st, const RegfLaunch &L, long long nblk) {
+    hipLaunchKernelGGL((k_regf_pass1<MFLAG, G>), dim3((unsigned)nblk), dim3(64),
+                       0, st, L.jp, L.desc, L.ncl, L.ntask, L.jst, L.xi, L.vi,
+                       L.h2, L.dtr, L.part, L.cnt, L.seg, L.err);
 }
 
 template <bool MFLAG, int G, int NR>
-static void launch_pass1_f64(hipStream_t st, const RegfLaunch &L, int n_iw,
-                             int nblk) {
-    hipLaunchKernelGGL((k_regf_pass1_f64<MFLAG, G, NR>), dim3(nblk), dim3(64),
-                       0, st, L.jp, L.nj, L.ntile, L.jst, L.xi, L.vi, L.h2,
-                       L.dtr, L.ni, n_iw, L.nslice, L.cap, L.part, L.cnt,
-                       L.seg, L.err);
+static void launch_pass1_f64(hipStream_t st, const RegfLaunch &L,
+                             long long nblk) {
+    hipLaunchKernelGGL((k_regf_pass1_f64<MFLAG, G, NR>), dim3((unsigned)nblk),
+                       dim3(64), 0, st, L.jp, L.desc, L.ncl, L.ntask, L.jst,
+                       L.xi, L.vi, L.h2, L.dtr, L.part, L.cnt, L.seg, L.err);
 }
 
 void launch_regf_pass1(hipStream_t st, const RegfLaunch &L) {
-    if (L.ni <= 0) return;
-    const int n_iw = cdiv(L.ni, 64);
-    const int ntask = n_iw * L.nslice;
-    const int nblk = 8 * ((ntask + 7) / 8);
+    if (L.ntask <= 0) return;
+    const long long nblk = 8 * ((L.ntask + 7) / 8);
     if (L.prec >= 64) {
         const int nr = L.prec - 64;   // 64: no Newton step, 65: one step
         if (L.group == 2) {
-            if (L.m_flag) nr ? launch_pass1_f64<true, 2, 1>(st, L, n_iw, nblk)
-                             : launch_pass1_f64<true, 2, 0>(st, L, n_iw, nblk);
-            else nr ? launch_pass1_f64<false, 2, 1>(st, L, n_iw, nblk)
-                    : launch_pass1_f64<false, 2, 0>(st, L, n_iw, nblk);
+            if (L.m_flag) nr ? launch_pass1_f64<true, 2, 1>(st, L, nblk)
+                             : launch_pass1_f64<true, 2, 0>(st, L, nblk);
+            else nr ? launch_pass1_f64<false, 2, 1>(st, L, nblk)
+                    : launch_pass1_f64<false, 2, 0>(st, L, nblk);
         } else {
-            if (L.m_flag) nr ? launch_pass1_f64<true, 4, 1>(st, L, n_iw, nblk)
-                             : launch_pass1_f64<true, 4, 0>(st, L, n_iw, nblk);
-            else nr ? launch_pass1_f64<false, 4, 1>(st, L, n_iw, nblk)
-                    : launch_pass1_f64<false, 4, 0>(st, L, n_iw, nblk);
+            if (L.m_flag) nr ? launch_pass1_f64<true, 4, 1>(st, L, nblk)
+                             : launch_pass1_f64<true, 4, 0>(st, L, nblk);
+            else nr ? launch_pass1_f64<false, 4, 1>(st, L, nblk)
+                    : launch_pass1_f64<false, 4, 0>(st, L, nblk);
         }
         return;
     }
     if (L.group == 2) {
-        if (L.m_flag) launch_pass1<true, 2>(st, L, n_iw, nblk);
-        else launch_pass1<false, 2>(st, L, n_iw, nblk);
+        if (L.m_flag) launch_pass1<true, 2>(st, L, nblk);
+        else launch_pass1<false, 2>(st, L, nblk);
     } else {
-        if (L.m_flag) launch_pass1<true, 4>(st, L, n_iw, nblk);
-        else launch_pass1<false, 4>(st, L, n_iw, nblk);
+        if (L.m_flag) launch_pass1<true, 4>(st, L, nblk);
+        else launch_pass1<false, 4>(st, L, nblk);
     }
 }
 
 void launch_regf_pass2(hipStream_t st, const RegfLaunch &L) {
-    if (L.ni <= 0) return;
+    if (L.nblk2 <= 0) return;
+    hipLaunchKernelGGL(k_regf_pass2, dim3((unsigned)L.nblk2), dim3(256), 0, st,
+                       L.jp, L.desc, L.ncl, L.xi, L.part, L.cnt, L.seg, L.acc,
+                       L.jrk, L.pot, L.nnb, L.nnid, L.list, L.off, L.total,
+                       L.list_cap, L.nnbmax, L.err);
+}
+
+int pass2_ipb(int ni) {
     // About 2048 workgroups for large calls (one list-space atomic per 16 i),
     // one i per wavefront for small ones.
-    const int ipb = std::max(4, std::min(kP2I, L.ni / 2048));
-    hipLaunchKernelGGL(k_regf_pass2, dim3(cdiv(L.ni, ipb)), dim3(256), 0, st,
-                       L.jp, L.xi, L.ni, L.nslice, L.cap, L.ntile, L.part,
-                       L.cnt, L.seg, L.acc, L.jrk, L.pot, L.nnb, L.nnid,
-                       L.list, L.off, L.total, L.list_cap, L.nnbmax, ipb, L.err);
+    return std::max(4, std::min(kP2I, ni / 2048));
 }
 
 void launch_pack_p(hipStream_t st, const double *m, const double *x, int n,

@@ -83,6 +83,16 @@ def _declare(lib: ctypes.CDLL) -> None:
         "gpunb_version": (ctypes.c_char_p, []),
         "gpunb_set_tuning": (ctypes.c_int, [ctypes.c_int, ctypes.c_double]),
         "gpunb_set_precision": (ctypes.c_int, [ctypes.c_int]),
+        "gpunb_ens_open": (ctypes.c_int, [ctypes.c_int, _ip]),
+        "gpunb_ens_close": (ctypes.c_int, []),
+        "gpunb_ens_send": (ctypes.c_int, [ctypes.c_int, _ip, _dp, _dp, _dp]),
+        "gpunb_ens_regf_dev": (ctypes.c_int,
+                               [ctypes.c_int, _ip] + [ctypes.c_void_p] * 7 +
+                               [ctypes.c_int, ctypes.c_int] +
+                               [ctypes.c_void_p] * 5 + [ctypes.c_longlong]),
+        "gpunb_ens_graph": (ctypes.c_int, [ctypes.c_int]),
+        "gpunb_ens_set_bucket_tiles": (ctypes.c_int, [ctypes.c_int, _ip]),
+        "gpunb_ens_pairs": (ctypes.c_longlong, []),
         "gpunb_timing": (ctypes.c_int, [ctypes.c_int]),
         "gpunb_timing_read": (ctypes.c_int,
                               [ctypes.POINTER(ctypes.c_double),
@@ -238,6 +248,38 @@ def gpunb_timing_read():
                                 ctypes.byref(w)) != 0:
         _check()
     return ms.value, n.value, w.value
+
+
+def gpunb_ens_open(nbmax) -> None:
+    nb = np.ascontiguousarray(nbmax, dtype=np.int32)
+    if load().gpunb_ens_open(nb.shape[0], _p(nb)) != 0:
+        _check()
+
+
+def gpunb_ens_close() -> None:
+    if load().gpunb_ens_close() != 0:
+        _check()
+
+
+def gpunb_ens_send(clusters) -> None:
+    """clusters: sequence of (m, x, v) per cluster."""
+    nj = np.array([len(c[0]) for c in clusters], dtype=np.int32)
+    m = _f64(np.concatenate([np.asarray(c[0], float) for c in clusters]))
+    x = _f64(np.concatenate([np.asarray(c[1], float).reshape(-1, 3) for c in clusters]))
+    v = _f64(np.concatenate([np.asarray(c[2], float).reshape(-1, 3) for c in clusters]))
+    if load().gpunb_ens_send(nj.shape[0], _p(nj), _p(m), _p(x), _p(v)) != 0:
+        _check()
+
+
+def gpunb_ens_graph(enable: bool) -> None:
+    if load().gpunb_ens_graph(1 if enable else 0) != 0:
+        _check()
+
+
+def gpunb_ens_set_bucket_tiles(tiles) -> None:
+    t = np.ascontiguousarray(tiles, dtype=np.int32)
+    if load().gpunb_ens_set_bucket_tiles(t.shape[0], _p(t)) != 0:
+        _check()
 
 
 def gpunb_sync() -> None:
