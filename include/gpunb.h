@@ -198,6 +198,41 @@ int gpunb_ens_set_bucket_tiles(int nbucket, const int *tiles);
 /* Pair interactions of the last ensemble launch (sum over clusters ni*nj). */
 long long gpunb_ens_pairs(void);
 
+/* ------------------------------------------------------------------------ */
+/* Learned neighbour radius (no reference counterpart: the reference sets RS */
+/* only by the Ahmad-Cohen volume rule, regcor_gpu.F:146-253, which stays    */
+/* the caller's fallback).  An MLP 8 -> 64 -> 64 -> 1 with SiLU activations, */
+/* BF16 weights and activations, FP32 accumulation, predicts log RS from 8   */
+/* per-particle features.  Parameter vector (FP32, gpunb_mlp_nparams()       */
+/* floats): W1[64][8] b1[64] W2[64][64] b2[64] W3[64] b3.  Inputs are        */
+/* normalised as (x - xmean) / xstd before the first layer; the output is    */
+/* yhat * ystd + ymean.  Host arrays; synchronous; 0 or -1 (error set).      */
+/* ------------------------------------------------------------------------ */
+
+int gpunb_mlp_nparams(void);
+
+/* Glorot-uniform weights from seed, zero biases, Adam state reset. */
+int gpunb_mlp_init(unsigned seed);
+
+int gpunb_mlp_set_norm(const float *xmean, const float *xstd, float ymean,
+                       float ystd);
+int gpunb_mlp_get_params(float *p);
+/* Sets the FP32 master parameters (and their BF16 copy); resets Adam. */
+int gpunb_mlp_set_params(const float *p);
+
+/* y[s] = network(x[s][0..7]) for n samples. */
+int gpunb_mlp_predict(int n, const float *x, float *y);
+
+/* Gradient of L = mean_s (yhat_s - (t_s - ymean)/ystd)^2 with respect to the
+ * parameters (BF16 rounding differentiated as identity), and L itself. */
+int gpunb_mlp_grad(int n, const float *x, const float *t, float *grad,
+                   double *loss);
+
+/* Adam (lr, beta 0.9/0.999, eps 1e-8) on minibatches of `batch` samples,
+ * reshuffled (seed) every epoch; *loss_out = mean loss of the last epoch. */
+int gpunb_mlp_train(int n, const float *x, const float *t, int epochs,
+                    int batch, float lr, unsigned seed, double *loss_out);
+
 /* Wait for all work on the library stream; returns 0 or -1 (error set). */
 int gpunb_sync(void);
 

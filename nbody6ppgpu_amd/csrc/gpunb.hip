@@ -1000,6 +1000,11 @@ void ens_regf_dev(int ncl, const int *ni, const double *h2, const double *dtr,
 }
 
 }  // namespace
+
+int run_guarded(const char *fn, const std::function<void()> &f) {
+    return guarded(fn, f);
+}
+
 }  // namespace gpunb
 
 using namespace gpunb;

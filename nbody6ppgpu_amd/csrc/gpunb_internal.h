@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <functional>
 
 namespace gpunb {
 
@@ -129,5 +130,11 @@ void launch_pack_p(hipStream_t st, const double *m, const double *x, int n,
 // Occupancy-derived resident-wave capacity of the regular-force kernel.
 int regf_resident_waves(int m_flag, int group, int prec);
 int pot_resident_waves();
+
+// Runs f under the library's error convention (the message goes to
+// gpunb_last_error(), the process aborts unless aborting is switched off):
+// 0 on success, -1 after an exception.  For entry points defined outside
+// gpunb.hip.
+int run_guarded(const char *fn, const std::function<void()> &f);
 
 }  // namespace gpunb

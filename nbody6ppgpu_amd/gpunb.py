@@ -26,6 +26,7 @@ LIB_PATH = os.path.join(_HERE, "lib", "libgpunb.so")
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
+_fp = ctypes.POINTER(ctypes.c_float)
 
 _lib: Optional[ctypes.CDLL] = None
 
@@ -93,6 +94,20 @@ def _declare(lib: ctypes.CDLL) -> None:
         "gpunb_ens_graph": (ctypes.c_int, [ctypes.c_int]),
         "gpunb_ens_set_bucket_tiles": (ctypes.c_int, [ctypes.c_int, _ip]),
         "gpunb_ens_pairs": (ctypes.c_longlong, []),
+        "gpunb_mlp_nparams": (ctypes.c_int, []),
+        "gpunb_mlp_init": (ctypes.c_int, [ctypes.c_uint]),
+        "gpunb_mlp_set_norm": (ctypes.c_int, [_fp, _fp, ctypes.c_float,
+                                              ctypes.c_float]),
+        "gpunb_mlp_get_params": (ctypes.c_int, [_fp]),
+        "gpunb_mlp_set_params": (ctypes.c_int, [_fp]),
+        "gpunb_mlp_predict": (ctypes.c_int, [ctypes.c_int, _fp, _fp]),
+        "gpunb_mlp_grad": (ctypes.c_int,
+                           [ctypes.c_int, _fp, _fp, _fp,
+                            ctypes.POINTER(ctypes.c_double)]),
+        "gpunb_mlp_train": (ctypes.c_int,
+                            [ctypes.c_int, _fp, _fp, ctypes.c_int,
+                             ctypes.c_int, ctypes.c_float, ctypes.c_uint,
+                             ctypes.POINTER(ctypes.c_double)]),
         "gpunb_timing": (ctypes.c_int, [ctypes.c_int]),
         "gpunb_timing_read": (ctypes.c_int,
                               [ctypes.POINTER(ctypes.c_double),

@@ -21,9 +21,19 @@
 // regular polynomial stays that of one consistent split (regcor_gpu.F's
 // gain/loss bookkeeping in its simplest form).  Steps follow Aarseth's
 // criterion with ETAI / ETAR, quantised to powers of two and kept
-// commensurate (block steps); the neighbour radius is adjusted towards NNBOPT
-// neighbours at every regular step, and an overflowing list is retried with a
-// smaller radius (util_gpu.F:70-101).
+// commensurate (block steps); an overflowing list is retried with a smaller
+// radius (util_gpu.F:70-101).
+//
+// Neighbour radius RS after each regular step (--rs):
+//   ac     the Ahmad-Cohen volume rule of regcor_gpu.F:146-253 (default);
+//   simple RS *= clamp((NNBOPT/NNB)^(1/3), 0.8, 1.25);
+//   mlp    the learned predictor (gpunb_mlp_predict, weights from --mlp),
+//          with the ac rule as the fallback for a body whose learned radius
+//          gave a list outside [NNBMIN, NNBMAX] or an overflow.
+// --log-train writes up to --log-max (feature, target) pairs drawn uniformly
+// (reservoir sampling) from all regular steps: the features of a body at one
+// regular step and the radius that would have given exactly NNBOPT
+// neighbours at its next one.
 //
 // Usage:  hermite_ac <in.bin> <t_end> [options]   (see parse_args)
 // in.bin: int32 n, then FP64 m[n], x[n][3], v[n][3].
@@ -35,7 +45,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <map>
+#include <random>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -62,7 +74,27 @@ struct Opts {
     int use_predict = 1;       // 1: device predictor; 0: host predict + send
     int prec = 0;              // gpunb_set_precision
     double rs0 = 0.0;          // initial RS (0: from nearest neighbours)
+    int rs_rule = 0;           // 0 ac, 1 simple, 2 mlp
+    double nnbmin = 0.0;       // 0: ZNBMIN = max(0.01 NNBMAX, 1) (input.F:122)
+    std::string mlp_path;      // weights file for --rs mlp
+    std::string log_path;      // training pairs out (--log-train)
+    long long log_max = 2000000;
+    unsigned log_seed = 12345;
 };
+
+// Features of the learned radius predictor (8 floats per body), all taken
+// after the body's regular step:
+//   0 log10 local mass density within RS, (m_i + sum_L m_j) / (4/3 pi RS^3)
+//   1 NNB / NNBOPT of the list just built
+//   2 log10 |a|        3 log10 |jerk|      (total force and jerk)
+//   4 log10 |x - density centre|           5 log10 (m / mean mass)
+//   6 log2 (1 / new regular step)          7 log10 RS used for the list
+// Target: log10 of the RS that gives NNBOPT neighbours at the next regular
+// step.  A logged record is 10 floats: features, target, censored flag (1
+// when the next list held fewer than NNBOPT, the target then being the
+// volume estimate RS (NNBOPT / NNB)^(1/3), at most 2 RS).
+constexpr int kNF = 8;
+constexpr int kRec = kNF + 2;
 
 struct Body {
     double m;
@@ -74,6 +106,9 @@ struct Body {
     double t0, dt, t0r, dtr;
     double rs;
     std::vector<int> list;     // neighbours (0-based, self excluded)
+    float feat[kNF];           // features at the last regular step
+    bool has_feat = false;
+    bool learned = false;      // rs came from the learned predictor
 };
 
 inline double norm3(const double *a) { return std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]); }
@@ -174,12 +209,30 @@ struct Run {
     // work counters
     long long reg_pairs = 0, irr_pairs = 0, reg_calls = 0, irr_steps = 0,
               reg_steps = 0, blocks = 0, overflows = 0;
-    double t_reg = 0.0, t_irr = 0.0, t_pred = 0.0, t_pot = 0.0;
+    double t_reg = 0.0, t_irr = 0.0, t_pred = 0.0, t_pot = 0.0, t_mlp = 0.0;
     // Block-step schedule: next time -> bodies due then (times are dyadic).
     std::map<double, std::vector<int>> due;
     // staging for the ABI
     std::vector<double> h2, dtrv, xi, vi, acc, jrk, pot, jm, jx, jv;
     std::vector<int> lst;
+    // body -> position in the current block's active / regular set (-1)
+    std::vector<int> rpos, rk;
+
+    // Cluster quantities of the Ahmad-Cohen radius rule, refreshed at every
+    // energy check as adjust.F does: density centre and core radius
+    // (Casertano & Hut 1985, 6th-neighbour densities from the lists),
+    // half-mass radius RSCALE about the density centre, and
+    // RSFAC = max(25/TCR, 1.5 VC/RSMIN) (adjust.F:220).
+    double rdens[3] = {0.0, 0.0, 0.0};
+    double rscale = 1.0, rc2 = 1.0, rsfac = 10.0, znbmin = 1.0, nnbmin = 1.0;
+    double tcr = 2.0 * std::sqrt(2.0), vc = std::sqrt(0.5), mmean = 1.0;
+    // Radius statistics.
+    long long learned_steps = 0, fallbacks = 0, nnb_dev_n = 0;
+    double nnb_dev_sum = 0.0;
+    // Training pairs (reservoir sample of all regular steps).
+    std::vector<float> logbuf;
+    long long log_seen = 0;
+    std::mt19937_64 log_rng;
 
     void abi_check() {
         char buf[512];
@@ -240,11 +293,13 @@ struct Run {
     // Retries overflowing lists with a smaller radius (util_gpu.F:70-101).
     void regular(const std::vector<int> &R, const std::vector<double> &X,
                  const std::vector<double> &V, std::vector<double> &FR,
-                 std::vector<double> &FRD, std::vector<std::vector<int>> &L) {
+                 std::vector<double> &FRD, std::vector<std::vector<int>> &L,
+                 std::vector<char> *ovf = nullptr) {
         const int nr = (int)R.size();
         FR.assign(3 * (size_t)nr, 0.0);
         FRD.assign(3 * (size_t)nr, 0.0);
         L.assign(nr, {});
+        if (ovf) ovf->assign(nr, 0);
         std::vector<int> todo(nr);
         for (int q = 0; q < nr; ++q) todo[q] = q;
         while (!todo.empty()) {
@@ -286,6 +341,7 @@ struct Run {
                 if (nnb < 0) {
                     // Overflow: shrink RS (NNBOPT/NNB)^(1/3) and redo.
                     overflows++;
+                    if (ovf) (*ovf)[r] = 1;
                     b.rs *= std::pow((double)o.nnbopt / (double)(-nnb), 1.0 / 3.0);
                     again.push_back(r);
                     continue;
@@ -327,9 +383,227 @@ struct Run {
         return std::max(q, dt);
     }
 
+    // Ahmad-Cohen neighbour-radius update after a regular step
+    // (regcor_gpu.F:146-253 with NB_FLAG = 1; no KS c.m. branch, no CHECKL).
+    // nnb: new list size, nnb0: previous list size, xi/vi at the step time,
+    // dtr: the regular step just completed, m3: min over the members of
+    // RIJ + RSDOT*DTR (used when nnb <= 3; +inf without members).
+    double ac_rs(double rs, int nnb, int nnb0, const double *xi,
+                 const double *vi, double dtr, double m3) const {
+        double d[3];
+        for (int c = 0; c < 3; ++c) d[c] = xi[c] - rdens[c];
+        const double ri2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        const double rh2 = rscale * rscale;
+        // NNB = 0 takes the reference's zero-neighbour path first; the
+        // volume rule below then sees NNB = 1.
+        const double fnb = std::max(nnb, 1);
+        const double fac = 1.0 + 0.1 * (o.nnbopt - fnb) / fnb;
+        double a3 = fnb * fac;
+        const double nbave = std::min(o.nnbopt + 150,
+                                      std::max(o.nnbmax - 50, (int)znbmin));
+        a3 = std::min(a3, 0.9 * nbave);
+        const int nbp = (int)a3;
+        if (ri2 > rh2) a3 *= std::max(rscale / std::sqrt(ri2), 0.1);
+        else a3 = std::max(a3, znbmin);
+        double a4 = a3 / fnb;
+        // Inertial factor against oscillations of RS.
+        if ((a3 - nnb0) * (a3 - fnb) < 0.0) a4 = std::sqrt(a4);
+        const double ridot = d[0] * vi[0] + d[1] * vi[1] + d[2] * vi[2];
+        if (ri2 > rc2 && ri2 < 100.0 * rh2) a4 *= 1.0 + ridot * dtr / ri2;
+        const double a5 = std::min(rsfac * dtr, 0.25);
+        a4 = std::min(std::max(a4 - 1.0, -a5), a5);
+        const double t3 = a4 / 3.0;
+        double a1 = 1.0 + t3 - t3 * t3;   // second-order cube root
+        if (nnb < znbmin && nbp > znbmin && a1 < 1.0) a1 = 1.05;
+        if (std::fabs(a1 - 1.0) > 0.003) {
+            if (rs * rs > ri2 && rs * rs < 2.0 * ri2 && nnb > 5) a1 = std::sqrt(a1);
+            rs *= a1;
+        }
+        if (nnb > nbave && ridot < 0.0) rs *= 1.0 - (nnb - nbave) / (3.0 * nnb);
+        if (nnb <= 3 && ri2 < 100.0 * rh2) rs = std::max(std::min(2.0 * rs, m3), 1.1 * rs);
+        return rs;
+    }
+
+    // The simple volume rule (previous default of this driver).
+    double simple_rs(double rs, int nnb) const {
+        const double nb = std::max<double>(1.0, (double)nnb);
+        double f = std::cbrt((double)o.nnbopt / nb);
+        return rs * std::min(std::max(f, 0.8), 1.25);
+    }
+
+    // Density centre, core radius, half-mass radius and RSFAC from the
+    // current state (every body at t0 == t) and lists.
+    void update_centre() {
+        std::vector<double> rho(n, 0.0);
+#pragma omp parallel for schedule(dynamic, 256)
+        for (int q = 0; q < n; ++q) {
+            const Body &b = B[q];
+            const size_t k = b.list.size();
+            if (k < 6) continue;
+            std::vector<std::pair<double, double>> dm(k);
+            for (size_t u = 0; u < k; ++u) {
+                const Body &bj = B[b.list[u]];
+                double r2 = 0.0;
+                for (int c = 0; c < 3; ++c) {
+                    const double e = bj.x0[c] - b.x0[c];
+                    r2 += e * e;
+                }
+                dm[u] = {r2, bj.m};
+            }
+            std::partial_sort(dm.begin(), dm.begin() + 6, dm.end());
+            double ms = 0.0;
+            for (int u = 0; u < 5; ++u) ms += dm[u].second;
+            const double r6 = std::sqrt(dm[5].first);
+            if (r6 > 0.0) rho[q] = ms / (r6 * r6 * r6);
+        }
+        double sw = 0.0, sx[3] = {0, 0, 0};
+        for (int q = 0; q < n; ++q) {
+            sw += rho[q];
+            for (int c = 0; c < 3; ++c) sx[c] += rho[q] * B[q].x0[c];
+        }
+        if (sw > 0.0)
+            for (int c = 0; c < 3; ++c) rdens[c] = sx[c] / sw;
+        double s2 = 0.0, sr = 0.0;
+        std::vector<std::pair<double, double>> rm(n);
+        double mt = 0.0;
+        for (int q = 0; q < n; ++q) {
+            double r2 = 0.0;
+            for (int c = 0; c < 3; ++c) {
+                const double e = B[q].x0[c] - rdens[c];
+                r2 += e * e;
+            }
+            s2 += rho[q] * rho[q];
+            sr += rho[q] * rho[q] * r2;
+            rm[q] = {r2, B[q].m};
+            mt += B[q].m;
+        }
+        if (s2 > 0.0) rc2 = sr / s2;
+        std::sort(rm.begin(), rm.end());
+        double cum = 0.0;
+        for (int q = 0; q < n; ++q) {
+            cum += rm[q].second;
+            if (cum >= 0.5 * mt) {
+                rscale = std::sqrt(rm[q].first);
+                break;
+            }
+        }
+        double rsmin = 1e30;
+        for (int q = 0; q < n; ++q)
+            if (!B[q].list.empty()) rsmin = std::min(rsmin, B[q].rs);
+        if (rsmin >= 1e30) rsmin = rscale;
+        rsfac = std::max(25.0 / tcr, 1.5 * vc / rsmin);
+    }
+
+    // Features of body b after its regular step (see kNF above); rsl = RS
+    // of the list just built, dtrn = the new regular step.
+    void features(const Body &b, double rsl, double dtrn, float *f) const {
+        double ml = b.m;
+        for (int j : b.list) ml += B[j].m;
+        const double vol = 4.0 / 3.0 * M_PI * rsl * rsl * rsl;
+        double d[3];
+        for (int c = 0; c < 3; ++c) d[c] = b.x0[c] - rdens[c];
+        auto lg = [](double v) { return (float)std::log10(std::max(v, 1e-30)); };
+        f[0] = lg(ml / vol);
+        f[1] = (float)((double)b.list.size() / o.nnbopt);
+        f[2] = lg(norm3(b.a));
+        f[3] = lg(norm3(b.j));
+        f[4] = lg(norm3(d));
+        f[5] = lg(b.m / mmean);
+        f[6] = (float)std::log2(1.0 / dtrn);
+        f[7] = lg(rsl);
+    }
+
+    // Target of the training pair closed by a new list L of body i at tn:
+    // log10 of the RS that gives exactly NNBOPT members, from the members'
+    // list-criterion distances min(r, r at t + dtr) (the engine's predicate);
+    // returns the censored flag.
+    int target(const std::vector<int> &L, double tn, const double *xi,
+               const double *vi, double dtr, double rs, float *t) const {
+        const size_t k = (size_t)o.nnbopt;
+        if (L.size() < k) {
+            const double f = std::min(std::cbrt((double)k / std::max<size_t>(L.size(), 1)), 2.0);
+            *t = (float)std::log10(rs * f);
+            return 1;
+        }
+        std::vector<double> c(L.size());
+        for (size_t u = 0; u < L.size(); ++u) {
+            double xj[3], vj[3], r2 = 0.0, p2 = 0.0;
+            predict_cubic(B[L[u]], tn, xj, vj);
+            for (int q = 0; q < 3; ++q) {
+                const double dx = xj[q] - xi[q], dv = vj[q] - vi[q];
+                r2 += dx * dx;
+                p2 += (dx + dtr * dv) * (dx + dtr * dv);
+            }
+            c[u] = std::max(std::min(r2, p2), 1e-300);
+        }
+        std::nth_element(c.begin(), c.begin() + (k - 1), c.end());
+        const double ck = c[k - 1];
+        double ck1 = rs * rs;
+        if (c.size() > k) ck1 = *std::min_element(c.begin() + k, c.end());
+        *t = (float)(0.25 * std::log10(ck * std::max(ck1, ck)));
+        return 0;
+    }
+
+    void log_add(const float *feat, float tgt, int cens) {
+        if (o.log_path.empty() || o.log_max <= 0) return;
+        long long slot = log_seen;
+        if (log_seen >= o.log_max) {
+            slot = (long long)(log_rng() % (unsigned long long)(log_seen + 1));
+            if (slot >= o.log_max) slot = -1;
+        } else {
+            logbuf.resize((size_t)(log_seen + 1) * kRec);
+        }
+        log_seen++;
+        if (slot < 0) return;
+        float *r = &logbuf[(size_t)slot * kRec];
+        for (int u = 0; u < kNF; ++u) r[u] = feat[u];
+        r[kNF] = tgt;
+        r[kNF + 1] = (float)cens;
+    }
+
+    void write_log() const {
+        if (o.log_path.empty()) return;
+        FILE *f = fopen(o.log_path.c_str(), "wb");
+        if (!f) {
+            perror(o.log_path.c_str());
+            exit(1);
+        }
+        fwrite(logbuf.data(), sizeof(float), logbuf.size(), f);
+        fclose(f);
+    }
+
+    // Weights file: gpunb_mlp_nparams() FP32 parameters, then xmean[8],
+    // xstd[8], ymean, ystd.
+    void load_mlp() {
+        const int np = gpunb_mlp_nparams();
+        std::vector<float> w((size_t)np + 2 * kNF + 2);
+        FILE *f = fopen(o.mlp_path.c_str(), "rb");
+        if (!f) {
+            perror(o.mlp_path.c_str());
+            exit(1);
+        }
+        const size_t got = fread(w.data(), sizeof(float), w.size(), f);
+        fclose(f);
+        if (got != w.size()) {
+            fprintf(stderr, "hermite_ac: %s: expected %zu floats\n",
+                    o.mlp_path.c_str(), w.size());
+            exit(1);
+        }
+        gpunb_mlp_set_params(w.data());
+        gpunb_mlp_set_norm(&w[np], &w[np + kNF], w[np + 2 * kNF],
+                           w[np + 2 * kNF + 1]);
+        abi_check();
+    }
+
     void init(const std::vector<double> &m, const std::vector<double> &x,
               const std::vector<double> &v) {
         B.resize(n);
+        rpos.assign(n, -1);
+        rk.assign(n, -1);
+        double mt = 0.0;
+        for (int q = 0; q < n; ++q) mt += m[q];
+        mmean = mt / n;
+        if (!o.log_path.empty()) log_rng.seed(o.log_seed);
         for (int q = 0; q < n; ++q) {
             Body &b = B[q];
             b = Body{};
@@ -444,10 +718,10 @@ struct Run {
         irr_steps += na;
 
         // Regular step for the bodies in R (engine call).
-        std::vector<int> rpos(n, -1);
         for (int k = 0; k < na; ++k) rpos[A[k]] = k;
         std::vector<double> FRN, FRDN;
         std::vector<std::vector<int>> LN;
+        std::vector<char> OVF;
         if (!R.empty()) {
             send_j(tn);
             std::vector<double> X(3 * R.size()), V(3 * R.size());
@@ -458,10 +732,9 @@ struct Run {
                     V[3 * k + c] = VP[3 * (size_t)a + c];
                 }
             }
-            regular(R, X, V, FRN, FRDN, LN);
+            regular(R, X, V, FRN, FRDN, LN, &OVF);
             reg_steps += (long long)R.size();
         }
-        std::vector<int> rk(n, -1);
         for (size_t k = 0; k < R.size(); ++k) rk[R[k]] = (int)k;
         // Irregular force of the new lists at tn (read-only pass: the
         // corrector below modifies bodies that are neighbours of others).
@@ -471,6 +744,13 @@ struct Run {
         // other way (regcor_gpu.F's neighbour-change corrections).
         std::vector<double> FIL(3 * R.size()), FIDL(3 * R.size());
         std::vector<double> DF2(3 * R.size(), 0.0), DF3(3 * R.size(), 0.0);
+        // Radius-rule inputs: min over members of RIJ + RSDOT*DTR for short
+        // lists (regcor_gpu.F:236-247) and the training target closed by
+        // the new list.
+        std::vector<double> M3(R.size(), 1e300);
+        std::vector<float> TG(R.size(), 0.0f);
+        std::vector<int> TC(R.size(), -1);
+        const bool logging = !o.log_path.empty();
         w0 = wtime();
 #pragma omp parallel for schedule(dynamic, 16)
         for (int q = 0; q < (int)R.size(); ++q) {
@@ -478,6 +758,21 @@ struct Run {
             const double *xi = &XP[3 * (size_t)a], *vi = &VP[3 * (size_t)a];
             irr_force(B, LN[q], tn, xi, vi, &FIL[3 * (size_t)q],
                       &FIDL[3 * (size_t)q]);
+            const Body &bi = B[R[q]];
+            if (LN[q].size() <= 3) {
+                for (int jb : LN[q]) {
+                    double xj[3], vj[3], r2 = 0.0, rv = 0.0;
+                    predict_cubic(B[jb], tn, xj, vj);
+                    for (int c = 0; c < 3; ++c) {
+                        r2 += (xi[c] - xj[c]) * (xi[c] - xj[c]);
+                        rv += (xi[c] - xj[c]) * (vi[c] - vj[c]);
+                    }
+                    const double rij = std::sqrt(r2);
+                    M3[q] = std::min(M3[q], rij + rv / rij * bi.dtr);
+                }
+            }
+            if (logging && bi.has_feat)
+                TC[q] = target(LN[q], tn, xi, vi, bi.dtr, bi.rs, &TG[q]);
             // Total acceleration and jerk of i at tn.
             double ai[3], ji[3];
             for (int c = 0; c < 3; ++c) {
@@ -517,7 +812,13 @@ struct Run {
                 }
             }
         }
-        for (size_t q = 0; q < R.size(); ++q) irr_pairs += (long long)LN[q].size();
+        for (size_t q = 0; q < R.size(); ++q) {
+            irr_pairs += (long long)LN[q].size();
+            nnb_dev_sum += std::fabs((double)LN[q].size() - o.nnbopt) / o.nnbopt;
+            if (TC[q] >= 0) log_add(B[R[q]].feat, TG[q], TC[q]);
+        }
+        nnb_dev_n += (long long)R.size();
+        std::vector<char> WANT(R.size(), 0), FELL(R.size(), 0);
 
 
 #pragma omp parallel for schedule(dynamic, 16)
@@ -529,6 +830,7 @@ struct Run {
             double fi2[3], fi3[3];
             hermite23(b.fi, b.fid, fin, fidn, h, fi2, fi3);
             const int r = rk[A[k]];
+            int nnb0 = 0;
             double newfi[3], newfid[3];
             for (int c = 0; c < 3; ++c) {
                 newfi[c] = fin[c];
@@ -573,6 +875,7 @@ struct Run {
                     newfi[c] = fiL2[c];
                     newfid[c] = fidL2[c];
                 }
+                nnb0 = (int)b.list.size();
                 b.list = LN[r];
                 b.t0r = tn;
             } else {
@@ -603,25 +906,67 @@ struct Run {
                 b.j[c] = b.fid[c] + frdt;
             }
             b.t0 = tn;
-            if (r >= 0) {
-                // Neighbour radius towards NNBOPT (volume scaling), then the
-                // new regular step.
-                const double nb = std::max<double>(1.0, (double)b.list.size());
-                double f = std::cbrt((double)o.nnbopt / nb);
-                f = std::min(std::max(f, 0.8), 1.25);
-                b.rs *= f;
-            }
             const double dtn = new_dt(b, tn, h, std::min(b.dtr, o.dtadj));
-            if (r >= 0) b.dtr = new_dtr(b, tn, b.dtr, dtn);
+            if (r >= 0) {
+                // New neighbour radius (rule above; the learned radii are
+                // filled in after this loop), then the new regular step.
+                const int nnb = (int)b.list.size();
+                const double rsl = b.rs, dtr_done = b.dtr;
+                if (o.rs_rule == 1) {
+                    b.rs = simple_rs(rsl, nnb);
+                } else {
+                    const double *xi = b.x0, *vi = b.v0;
+                    b.rs = ac_rs(rsl, nnb, nnb0, xi, vi, dtr_done, M3[r]);
+                }
+                b.dtr = new_dtr(b, tn, dtr_done, dtn);
+                features(b, rsl, b.dtr, b.feat);
+                b.has_feat = true;
+                if (o.rs_rule == 2) {
+                    const bool bad = OVF[r] || nnb < nnbmin || nnb > o.nnbmax;
+                    FELL[r] = b.learned && bad;
+                    WANT[r] = !FELL[r];
+                }
+                b.learned = false;
+            }
             // Irregular step never beyond the next regular time.
             double d = dtn;
             while (tn + d > b.t0r + b.dtr && d > o.dtmin) d *= 0.5;
             b.dt = d;
         }
         t_irr += wtime() - w0;
+        if (o.rs_rule == 2 && !R.empty()) learned_radii(R, WANT, FELL);
         for (int q : A) due[B[q].t0 + B[q].dt].push_back(q);
+        for (int q : A) rpos[q] = -1;
+        for (int q : R) rk[q] = -1;
         jpset(A);
         t = tn;
+    }
+
+    // Learned radii for the regular bodies flagged in want (one batched
+    // inference); a non-finite prediction keeps the rule's radius.
+    void learned_radii(const std::vector<int> &R, const std::vector<char> &want,
+                       const std::vector<char> &fell) {
+        std::vector<int> idx;
+        for (size_t q = 0; q < R.size(); ++q) {
+            if (fell[q]) fallbacks++;
+            if (want[q]) idx.push_back(R[q]);
+        }
+        if (idx.empty()) return;
+        const int k = (int)idx.size();
+        std::vector<float> F((size_t)k * kNF), Y(k);
+        for (int u = 0; u < k; ++u)
+            std::copy(B[idx[u]].feat, B[idx[u]].feat + kNF, &F[(size_t)u * kNF]);
+        double w0 = wtime();
+        gpunb_mlp_predict(k, F.data(), Y.data());
+        abi_check();
+        t_mlp += wtime() - w0;
+        for (int u = 0; u < k; ++u) {
+            const double rs = std::pow(10.0, (double)Y[u]);
+            if (!std::isfinite(rs) || rs <= 0.0) continue;
+            B[idx[u]].rs = rs;
+            B[idx[u]].learned = true;
+            learned_steps++;
+        }
     }
 };
 
@@ -630,7 +975,9 @@ void parse_args(int argc, char **argv, Opts &o, std::string &in) {
         fprintf(stderr,
                 "usage: %s in.bin t_end [--etai x] [--etar x] [--dtadj x] "
                 "[--nnbopt k] [--lmax k] [--nnbmax k] [--send] [--prec p] "
-                "[--rs0 x]\n",
+                "[--rs0 x] [--rs ac|simple|mlp] [--mlp weights] "
+                "[--nnbmin x] [--log-train file] [--log-max n] "
+                "[--log-seed s]\n",
                 argv[0]);
         exit(1);
     }
@@ -648,6 +995,21 @@ void parse_args(int argc, char **argv, Opts &o, std::string &in) {
         else if (s == "--send") o.use_predict = 0;
         else if (s == "--prec") o.prec = atoi(nxt());
         else if (s == "--rs0") o.rs0 = atof(nxt());
+        else if (s == "--rs") {
+            const std::string r = nxt();
+            if (r == "ac") o.rs_rule = 0;
+            else if (r == "simple") o.rs_rule = 1;
+            else if (r == "mlp") o.rs_rule = 2;
+            else {
+                fprintf(stderr, "unknown --rs %s\n", r.c_str());
+                exit(1);
+            }
+        }
+        else if (s == "--mlp") o.mlp_path = nxt();
+        else if (s == "--nnbmin") o.nnbmin = atof(nxt());
+        else if (s == "--log-train") o.log_path = nxt();
+        else if (s == "--log-max") o.log_max = atoll(nxt());
+        else if (s == "--log-seed") o.log_seed = (unsigned)atol(nxt());
         else {
             fprintf(stderr, "unknown option %s\n", s.c_str());
             exit(1);
@@ -696,10 +1058,29 @@ int main(int argc, char **argv) {
     int nbmax = n + 10, irank = 0;
     gpunb_open_(&nbmax, &irank);
     R.abi_check();
+    if (o.rs_rule == 2) {
+        if (o.mlp_path.empty()) {
+            fprintf(stderr, "--rs mlp needs --mlp weights\n");
+            return 1;
+        }
+        R.load_mlp();
+    }
     double w0 = wtime();
     R.init(m, x, v);
     double ek, ep;
     const double e0 = R.energy(&ek, &ep);
+    {
+        // Crossing time and velocity scale of scale.F, then the cluster
+        // quantities of the radius rule.
+        double mt = 0.0;
+        for (int q = 0; q < n; ++q) mt += m[q];
+        const double e2 = 2.0 * std::fabs(e0);
+        R.tcr = std::pow(mt, 2.5) / std::pow(e2, 1.5);
+        R.vc = std::sqrt(e2 / mt);
+        R.znbmin = std::max(0.01 * o.nnbmax, 1.0);
+        R.nnbmin = o.nnbmin > 0.0 ? o.nnbmin : R.znbmin;
+        R.update_centre();
+    }
     printf("{\"t\": 0.0, \"E\": %.15e, \"EK\": %.15e, \"EP\": %.15e, \"dE_E\": 0.0}\n",
            e0, ek, ep);
     fflush(stdout);
@@ -715,16 +1096,25 @@ int main(int argc, char **argv) {
                    "\"dE_E\": %.6e, \"blocks\": %lld, \"irr_steps\": %lld, "
                    "\"reg_steps\": %lld, \"reg_calls\": %lld, \"reg_pairs\": %lld, "
                    "\"irr_pairs\": %lld, \"overflows\": %lld, \"nnb_mean\": %.2f, "
+                   "\"nnb_dev\": %.4f, \"rs_rule\": \"%s\", "
+                   "\"learned_steps\": %lld, \"fallbacks\": %lld, "
+                   "\"log_seen\": %lld, \"rscale\": %.4f, \"rcore\": %.4f, "
                    "\"t_reg\": %.3f, \"t_irr\": %.3f, \"t_pred\": %.3f, "
-                   "\"t_pot\": %.3f, \"wall\": %.3f}\n",
+                   "\"t_pot\": %.3f, \"t_mlp\": %.3f, \"wall\": %.3f}\n",
                    R.t, e, ek, ep, (e - e0) / std::fabs(e0), R.blocks,
                    R.irr_steps, R.reg_steps, R.reg_calls, R.reg_pairs,
-                   R.irr_pairs, R.overflows, (double)nnbs / n, R.t_reg, R.t_irr,
-                   R.t_pred, R.t_pot, wall);
+                   R.irr_pairs, R.overflows, (double)nnbs / n,
+                   R.nnb_dev_n ? R.nnb_dev_sum / R.nnb_dev_n : 0.0,
+                   o.rs_rule == 0 ? "ac" : o.rs_rule == 1 ? "simple" : "mlp",
+                   R.learned_steps, R.fallbacks, R.log_seen, R.rscale,
+                   std::sqrt(R.rc2), R.t_reg, R.t_irr, R.t_pred, R.t_pot,
+                   R.t_mlp, wall);
             fflush(stdout);
             next += o.dtadj;
+            R.update_centre();
         }
     }
+    R.write_log();
     gpunb_close_();
     return 0;
 }

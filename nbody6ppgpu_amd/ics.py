@@ -6,6 +6,8 @@ E = -1/4 for the virialised models):
 * ``plummer``  -- Plummer sphere by the Aarseth-Henon-Wielen recipe (the same
   recipe as the reference's src/Main/setup.F:55-104), scaled analytically to
   E = -1/4 (scale radius 3*pi/16).
+* ``plummer_segregated`` -- Plummer sphere with full radial mass segregation
+  (masses ranked by binding energy), virial ratio 1/2, E = -1/4.
 * ``king``     -- King (1966) model of central potential W0: the Poisson
   equation of the lowered isothermal distribution is integrated numerically,
   radii are drawn from the cumulative mass, speeds from the local distribution
@@ -84,6 +86,41 @@ def plummer(n: int, seed: int = 1, m=None) -> Cluster:
     x -= (m[:, None] * x).sum(0)
     v -= (m[:, None] * v).sum(0)
     return Cluster(m, x, v, {"model": "plummer", "seed": seed})
+
+
+def plummer_segregated(n: int, seed: int = 1, m=None) -> Cluster:
+    """Plummer sphere with full radial mass segregation (Baumgardt, De Marchi
+    & Kroupa 2008, ApJ 685, 247): phase-space points of an equal-mass
+    Plummer model, masses (Kroupa IMF unless ``m`` is given) assigned in
+    order of specific binding energy, the heaviest to the most bound.  Then
+    velocities are scaled to virial ratio 1/2 and lengths to E = -1/4, with
+    the potential energy from the shell theorem about the centre of mass
+    (exact for a spherical distribution; its error for the sampled cluster is
+    of the order of the sampling noise)."""
+    c = plummer(n, seed=seed)
+    if m is None:
+        m = kroupa(n, seed=seed + 1)
+    m = np.sort(np.asarray(m, dtype=np.float64))[::-1]
+    m = m / m.sum()
+    a = 3.0 * np.pi / 16.0
+    r = np.sqrt(np.sum(c.x * c.x, axis=1))
+    eps = 0.5 * np.sum(c.v * c.v, axis=1) - 1.0 / np.sqrt(r * r + a * a)
+    order = np.argsort(eps, kind="stable")
+    x, v = c.x[order].copy(), c.v[order].copy()
+    x -= (m[:, None] * x).sum(0)
+    v -= (m[:, None] * v).sum(0)
+    r = np.sqrt(np.sum(x * x, axis=1))
+    srt = np.argsort(r, kind="stable")
+    rs_, ms_ = r[srt], m[srt]
+    inner = np.cumsum(ms_) - ms_                  # mass strictly inside
+    outer = np.cumsum((ms_ / rs_)[::-1])[::-1] - ms_ / rs_
+    pe = -0.5 * np.sum(ms_ * (inner / rs_ + outer))   # 1/2 sum m_i phi_i
+    ke = 0.5 * np.sum(m * np.sum(v * v, axis=1))
+    v *= np.sqrt(0.5 * abs(pe) / ke)
+    s = -2.0 * pe                                 # E = pe/2 -> -1/4
+    x *= s
+    v /= np.sqrt(s)
+    return Cluster(m, x, v, {"model": "plummer_segregated", "seed": seed})
 
 
 def _king_profile(w0: float, nr: int = 4000):
