@@ -106,6 +106,12 @@ def main():
                     help="skip the Hermite/Ahmad-Cohen integration leg")
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the ensemble (64 clusters) leg")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned neighbour-radius leg")
+    ap.add_argument("--learned-n", type=int, default=262144,
+                    help="bodies of the learned-radius leg (the full "
+                         "workload is 1048576: about 6 minutes)")
+    ap.add_argument("--learned-t", type=float, default=0.25)
     args = ap.parse_args()
     if args.gpus != 1:
         sys.exit("bench.py runs on one GPU only: this build measures the "
@@ -256,6 +262,23 @@ def main():
                       "bucket_tiles": {f"2^{b}": t for b, t in enumerate(tiles) if t}},
         }
 
+    # Learned neighbour radius: mass-segregated Plummer with a Kroupa IMF;
+    # 2M (at most) training pairs logged from a plain run, the network
+    # trained on the GPU, then an independent realisation run with the
+    # reference's Ahmad-Cohen rule, the plain volume rule and the learned
+    # radius (Ahmad-Cohen fallback), each for learned_t N-body time units.
+    learned = None
+    if not args.no_learned:
+        from nbody6ppgpu_amd import rsnet
+        log(f"bench: learned neighbour radius, N={args.learned_n}")
+        ex = rsnet.experiment(n=args.learned_n, t_log=args.learned_t,
+                              t_run=args.learned_t, epochs=20, log=log)
+        learned = {
+            "workload": "mass-segregated Plummer, Kroupa IMF 0.08-150 Msun, "
+                        f"N={args.learned_n}, {args.learned_t} N-body time "
+                        "units per run; MLP 8-64-64-1 BF16/FP32 on MFMA",
+            "fit": ex["fit"], "runs": ex["runs"]}
+
     traffic = load_traffic()
     out = {
         "metric": "regular-force pair interactions/s at N=128k, % FP32 peak; "
@@ -299,6 +322,7 @@ def main():
         "abi_ni1024_pairs_per_s": abi_rate,
         "hermite_1tnb": hermite,
         "ensemble": ensemble,
+        "learned_rs": learned,
     }
     print(json.dumps(out), flush=True)
 

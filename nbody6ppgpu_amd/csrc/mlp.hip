@@ -3,17 +3,31 @@
 // Inference and training run on the GPU with these kernels; the plain
 // Ahmad-Cohen radius update stays the fallback in the caller.
 //
-// Arithmetic: weights and activations are rounded to BF16 (round to nearest
-// even) before every multiply; products are accumulated in FP32.  Training
-// keeps FP32 master weights, FP32 gradients and an FP32 Adam state; the BF16
+// Arithmetic: weights, activations and back-propagated deltas enter the
+// products as BF16 (round to nearest even); products are accumulated in
+// FP32 on the matrix cores (v_mfma_f32_16x16x16_bf16).  Bias additions, SiLU
+// and the bias / output-layer gradients are FP32 vector work.  Training keeps
+// FP32 master weights, FP32 gradients and an FP32 Adam state; the BF16
 // weights are re-derived from the masters after every step.
 //
+// MFMA mapping.  A wavefront evaluates 16 samples.  Every layer is computed
+// transposed, Z^T = W H^T: the weights are the A operand (16 output units x
+// 16 inputs per instruction), the activations the B operand (16 inputs x 16
+// samples).  The 16x16 FP32 result tile leaves lane l holding sample l%16 and
+// units 4(l/16)..4(l/16)+3 of the tile, which is exactly the B-operand layout
+// of the next layer's k-step, so activations never leave registers.  The
+// backward delta d1^T = W2^T d2^T reuses the same layout with W2^T as A.
+// Weight gradients (sums over samples) need the samples on the k dimension:
+// the wave stages its deltas and activations through LDS and reads them back
+// transposed.
+//
 // Kernels
-//   k_mlp_forward   one sample per thread, weights in LDS (BF16).
-//   k_mlp_fwdbwd    forward + backward of 64 samples per workgroup, per-block
-//                   gradient sums (activations and deltas staged in LDS).
-//   k_mlp_reduce    sums the per-block gradients in a fixed order
-//                   (deterministic) and applies Adam (or only stores the sum).
+//   k_mlp_forward   inference; 4 waves x 16 samples per workgroup, loop.
+//   k_mlp_fwdbwd    forward + backward, per-wave gradient sums accumulated
+//                   in registers over a workgroup-strided loop, one partial
+//                   row per wave.
+//   k_mlp_reduce    sums the partial rows in a fixed order (deterministic)
+//                   and applies Adam (or only stores the sum).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -37,16 +51,27 @@ constexpr int NI = 8, NH = 64;
 constexpr int OW1 = 0, OB1 = OW1 + NH * NI, OW2 = OB1 + NH,
               OB2 = OW2 + NH * NH, OW3 = OB2 + NH, OB3 = OW3 + NH,
               NP = OB3 + 1;
-constexpr int BS = 64;  // samples per training workgroup
+constexpr int kWaves = 4;               // waves per workgroup
+constexpr int kGroup = 16 * kWaves;     // samples per workgroup iteration
+constexpr int kMaxBlocks = 256;         // training workgroups (partial rows / 4)
 
-__device__ __host__ inline float bf16r(float x) {
-    // Round to nearest even to BF16 and back to FP32.
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __host__ inline uint16_t bf16_bits(float x) {
+    // Round to nearest even; upper 16 bits of the FP32 pattern.
     uint32_t u;
     memcpy(&u, &x, 4);
-    u = (u + 0x7FFFu + ((u >> 16) & 1u)) & 0xFFFF0000u;
-    float y;
-    memcpy(&y, &u, 4);
-    return y;
+    u = u + 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+__device__ inline float bf16_val(uint16_t h) {
+    return __uint_as_float((uint32_t)h << 16);
+}
+
+__device__ inline f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
 __device__ inline float silu(float z) { return z / (1.0f + __expf(-z)); }
@@ -58,182 +83,278 @@ __device__ inline float dsilu(float z) {
 struct Norm {
     float xm[NI], xs[NI], ym, ys;
 };
-static_assert(BS == NH, "k_mlp_fwdbwd maps one lane to one hidden unit");
 
-// Weights as BF16-rounded FP32 in LDS (exact BF16 values, FP32 storage).
-__device__ inline void load_weights(const float *__restrict__ wb, float *sw) {
-    for (int k = threadIdx.x; k < NP; k += blockDim.x) sw[k] = wb[k];
-    __syncthreads();
-}
+// Per-lane weights of one wave.  Lane l (r = l%16, q = l/16):
+//   w1[T]      A operand of layer 1, output tile T: W1[16T+r][4q..4q+3]
+//              (inputs 8..15 are zero padding, q >= 2);
+//   w2[T][t]   A operand of layer 2: W2[16T+r][16t+4q..+3];
+//   b1/b2/w3   FP32 values for this lane's units u = 16T+4q+v (result layout).
+struct FwdWeights {
+    bf16x4 w1[4];
+    bf16x4 w2[4][4];
+    float b1[4][4], b2[4][4], w3[4][4], b3;
+};
 
-// Layer 1 of one sample: h1 = bf16(silu(W1 xh + b1)); ds1 = silu'(z1).
-__device__ inline void layer1(const float *sw, const float *xh, float *h1,
-                              float *ds1) {
+__device__ inline void load_fwd(const uint16_t *__restrict__ wh, int l,
+                                FwdWeights &W) {
+    const int r = l & 15, q = l >> 4;
 #pragma unroll
-    for (int a = 0; a < NH; ++a) {
-        float acc = sw[OB1 + a];
+    for (int T = 0; T < 4; ++T) {
 #pragma unroll
-        for (int k = 0; k < NI; ++k) acc = fmaf(sw[OW1 + a * NI + k], xh[k], acc);
-        h1[a] = bf16r(silu(acc));
-        if (ds1) ds1[a] = dsilu(acc);
+        for (int j = 0; j < 4; ++j)
+            W.w1[T][j] = q < 2 ? (short)wh[OW1 + (16 * T + r) * NI + 4 * q + j] : 0;
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                W.w2[T][t][j] = (short)wh[OW2 + (16 * T + r) * NH + 16 * t + 4 * q + j];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int u = 16 * T + 4 * q + v;
+            W.b1[T][v] = bf16_val(wh[OB1 + u]);
+            W.b2[T][v] = bf16_val(wh[OB2 + u]);
+            W.w3[T][v] = bf16_val(wh[OW3 + u]);
+        }
     }
+    W.b3 = bf16_val(wh[OB3]);
 }
 
-// Layer 2 unit b of one sample: z2 = W2[b,:] h1 + b2[b].
-__device__ inline float layer2(const float *sw, const float *h1, int b) {
-    float acc = sw[OB2 + b];
+// Layer-1 B operand of sample `sidx` (lanes q < 2 hold inputs 4q..4q+3).
+__device__ inline bf16x4 input_operand(const Norm &nm, const float *__restrict__ x,
+                                       int sidx, bool valid, int q, float *xh) {
+    bf16x4 b = {0, 0, 0, 0};
 #pragma unroll
-    for (int a = 0; a < NH; ++a) acc = fmaf(sw[OW2 + b * NH + a], h1[a], acc);
-    return acc;
+    for (int v = 0; v < 4; ++v) xh[v] = 0.0f;
+    if (valid && q < 2) {
+        const float4 xv = *reinterpret_cast<const float4 *>(x + (size_t)NI * sidx + 4 * q);
+        const float xx[4] = {xv.x, xv.y, xv.z, xv.w};
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const uint16_t h = bf16_bits((xx[v] - nm.xm[4 * q + v]) / nm.xs[4 * q + v]);
+            b[v] = (short)h;
+            xh[v] = bf16_val(h);
+        }
+    }
+    return b;
 }
 
-__device__ inline void norm_in(const Norm &nm, const float *x, float *xh) {
+// Forward of the wave's 16 samples.  Outputs (result layout, lane = sample
+// l%16, units 16T+4q+v): h1 as the layer-2 B operands, silu'(z1), h2,
+// silu'(z2); returns the network output for sample l%16 (in every lane).
+__device__ inline float forward16(const FwdWeights &W, bf16x4 xb, bf16x4 *h1b,
+                                  float (*ds1)[4], float (*h2)[4],
+                                  float (*ds2)[4]) {
+    const f32x4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-    for (int k = 0; k < NI; ++k) xh[k] = bf16r((x[k] - nm.xm[k]) / nm.xs[k]);
+    for (int T = 0; T < 4; ++T) {
+        const f32x4 z = mfma16(W.w1[T], xb, zero);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const float zz = z[v] + W.b1[T][v];
+            h1b[T][v] = (short)bf16_bits(silu(zz));
+            if (ds1) ds1[T][v] = dsilu(zz);
+        }
+    }
+    float part = 0.0f;
+#pragma unroll
+    for (int T = 0; T < 4; ++T) {
+        f32x4 z = zero;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) z = mfma16(W.w2[T][t], h1b[t], z);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const float zz = z[v] + W.b2[T][v];
+            const float h = bf16_val(bf16_bits(silu(zz)));
+            if (h2) h2[T][v] = h;
+            if (ds2) ds2[T][v] = dsilu(zz);
+            part = fmaf(W.w3[T][v], h, part);
+        }
+    }
+    // Sum over the four lanes holding this sample (q = 0..3).
+    part += __shfl_xor(part, 16);
+    part += __shfl_xor(part, 32);
+    return part + W.b3;
 }
 
 __global__ __launch_bounds__(256) void k_mlp_forward(
-    int n, const float *__restrict__ x, const float *__restrict__ wb, Norm nm,
-    float *__restrict__ y) {
-    __shared__ float sw[NP];
-    load_weights(wb, sw);
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    float xh[NI], h1[NH];
-    norm_in(nm, x + (size_t)NI * s, xh);
-    layer1(sw, xh, h1, nullptr);
-    float out = sw[OB3];
-#pragma unroll
-    for (int b = 0; b < NH; ++b)
-        out = fmaf(sw[OW3 + b], bf16r(silu(layer2(sw, h1, b))), out);
-    y[s] = out * nm.ys + nm.ym;
+    int n, const float *__restrict__ x, const uint16_t *__restrict__ wh,
+    Norm nm, float *__restrict__ y) {
+    const int l = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r = l & 15, q = l >> 4;
+    FwdWeights W;
+    load_fwd(wh, l, W);
+    const int ngroups = (n + 15) / 16;
+    for (int g = blockIdx.x * kWaves + wave; g < ngroups; g += gridDim.x * kWaves) {
+        const int sidx = 16 * g + r;
+        const bool valid = sidx < n;
+        float xh[4];
+        const bf16x4 xb = input_operand(nm, x, sidx, valid, q, xh);
+        bf16x4 h1b[4];
+        const float out = forward16(W, xb, h1b, nullptr, nullptr, nullptr);
+        if (valid && q == 0) y[sidx] = out * nm.ys + nm.ym;
+    }
 }
 
 // Forward + backward of the samples idx[base .. base+count) (idx may be null:
-// identity).  Loss = mean over the batch of (yhat - t_normalised)^2, yhat and
-// the target in normalised units.  The derivative of the BF16 rounding is
-// taken as 1 (straight-through).  Writes this block's gradient sums to
-// gpart[blockIdx.x][NP] and its summed squared error to lpart[blockIdx.x].
-constexpr int LP = NH + 1;  // padded LDS row (no bank conflicts per column)
-__global__ __launch_bounds__(BS) void k_mlp_fwdbwd(
+// identity).  Loss = mean over the batch of (yhat - t_normalised)^2.  The
+// derivative of the BF16 rounding is taken as 1 (straight-through).  Deltas
+// d2 and d1 enter the MFMA products (d1 = W2^T d2, the weight gradients) as
+// BF16; the bias and output-layer gradients use them in FP32.  Each wave
+// accumulates its gradient sums over its samples and writes them, with its
+// summed squared error, to row blockIdx.x*kWaves + wave of gpart / lpart.
+constexpr int LDS_ROW = NH + 8;   // bf16 per sample row (16-byte padding)
+__global__ __launch_bounds__(256) void k_mlp_fwdbwd(
     int count, int base, const int *__restrict__ idx,
     const float *__restrict__ x, const float *__restrict__ t,
-    const float *__restrict__ wb, Norm nm, float inv_batch,
+    const uint16_t *__restrict__ wh, Norm nm, float inv_batch,
     float *__restrict__ gpart, float *__restrict__ lpart) {
-    __shared__ float sw[NP];
-    __shared__ float sh1[BS * LP], sh2[BS * LP], sd1[BS * LP], sd2[BS * LP];
-    __shared__ float sxh[BS * (NI + 1)];
-    __shared__ float sd3[BS];
-    load_weights(wb, sw);
-    const int r = threadIdx.x;
-    const int q = blockIdx.x * BS + r;
-    float loss = 0.0f;
-    if (q < count) {
-        // Per-sample vectors live in this lane's LDS rows; only h1 and d2
-        // are held in registers (fully unrolled inner loops), so the outer
-        // loops stay rolled and the weights are not hoisted into registers.
-        const int s = idx ? idx[base + q] : base + q;
-        float xh[NI], h1[NH];
-        norm_in(nm, x + (size_t)NI * s, xh);
+    __shared__ uint16_t sH1[kWaves][16 * LDS_ROW];
+    __shared__ uint16_t sD2[kWaves][16 * LDS_ROW];
+    __shared__ uint16_t sD1[kWaves][16 * LDS_ROW];
+    __shared__ uint16_t sXH[kWaves][16 * 16];
+    const int l = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r = l & 15, q = l >> 4;
+    FwdWeights W;
+    load_fwd(wh, l, W);
+    // A operand of d1^T = W2^T d2^T: lane holds W2[16t+4q+j][16T+r].
+    bf16x4 w2t[4][4];
 #pragma unroll
-        for (int a = 0; a < NH; ++a) {
-            float acc = sw[OB1 + a];
+    for (int T = 0; T < 4; ++T)
 #pragma unroll
-            for (int k = 0; k < NI; ++k) acc = fmaf(sw[OW1 + a * NI + k], xh[k], acc);
-            h1[a] = bf16r(silu(acc));
-            sh1[r * LP + a] = h1[a];
-            sd1[r * LP + a] = dsilu(acc);      // silu'(z1), replaced by d1
-        }
-        float out = sw[OB3];
-#pragma unroll 1
-        for (int b = 0; b < NH; ++b) {
-            const float z = layer2(sw, h1, b);
-            const float h = bf16r(silu(z));
-            sd2[r * LP + b] = dsilu(z);        // silu'(z2), replaced by d2
-            sh2[r * LP + b] = h;
-            out = fmaf(sw[OW3 + b], h, out);
-        }
-        const float e = out - (t[s] - nm.ym) / nm.ys;
-        loss = e * e;
-        const float d3 = 2.0f * e * inv_batch;
-        sd3[r] = d3;
-        float d2[NH];
+        for (int tt = 0; tt < 4; ++tt)
 #pragma unroll
-        for (int b = 0; b < NH; ++b) {
-            d2[b] = d3 * sw[OW3 + b] * sd2[r * LP + b];
-            sd2[r * LP + b] = d2[b];
-        }
-#pragma unroll 1
-        for (int a = 0; a < NH; ++a) {
-            float acc = 0.0f;
+            for (int j = 0; j < 4; ++j)
+                w2t[T][tt][j] = (short)wh[OW2 + (16 * tt + 4 * q + j) * NH + 16 * T + r];
+    const f32x4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+    f32x4 gw2[4][4], gw1[4];
+    float gb1[4][4], gb2[4][4], gw3[4][4], gb3 = 0.0f, loss = 0.0f;
 #pragma unroll
-            for (int b = 0; b < NH; ++b) acc = fmaf(sw[OW2 + b * NH + a], d2[b], acc);
-            sd1[r * LP + a] = acc * sd1[r * LP + a];
-        }
+    for (int T = 0; T < 4; ++T) {
+        gw1[T] = zero;
 #pragma unroll
-        for (int k = 0; k < NI; ++k) sxh[r * (NI + 1) + k] = xh[k];
-    } else {
-        sd3[r] = 0.0f;
-        for (int b = 0; b < NH; ++b) {
-            sd2[r * LP + b] = 0.0f;
-            sh2[r * LP + b] = 0.0f;
-            sd1[r * LP + b] = 0.0f;
-            sh1[r * LP + b] = 0.0f;
+        for (int v = 0; v < 4; ++v) {
+            gw2[T][v] = zero;
+            gb1[T][v] = gb2[T][v] = gw3[T][v] = 0.0f;
         }
-        for (int k = 0; k < NI; ++k) sxh[r * (NI + 1) + k] = 0.0f;
     }
-    __syncthreads();
-    // Per-block sums over the BS samples; thread r owns unit r of both
-    // hidden layers (BS == NH).
-    float *g = gpart + (size_t)blockIdx.x * NP;
-    const int u = r;
-    {
-        float gb2 = 0.0f, gw3 = 0.0f, gb1 = 0.0f;
-        for (int s2 = 0; s2 < BS; ++s2) {
-            gb2 += sd2[s2 * LP + u];
-            gw3 = fmaf(sd3[s2], sh2[s2 * LP + u], gw3);
-            gb1 += sd1[s2 * LP + u];
+    uint16_t *h1s = sH1[wave], *d2s = sD2[wave], *d1s = sD1[wave], *xs = sXH[wave];
+    const int nbg = (count + kGroup - 1) / kGroup;
+    // Uniform trip count across the workgroup (barriers inside the loop).
+    for (int bg = blockIdx.x; bg < nbg; bg += gridDim.x) {
+        const int qi = bg * kGroup + 16 * wave + r;
+        const bool valid = qi < count;
+        const int sidx = valid ? (idx ? idx[base + qi] : base + qi) : 0;
+        float xh[4];
+        const bf16x4 xb = input_operand(nm, x, sidx, valid, q, xh);
+        bf16x4 h1b[4];
+        float ds1[4][4], h2[4][4], ds2[4][4];
+        const float out = forward16(W, xb, h1b, ds1, h2, ds2);
+        float d3 = 0.0f;
+        if (valid) {
+            const float e = out - (t[sidx] - nm.ym) / nm.ys;
+            d3 = 2.0f * e * inv_batch;
+            if (q == 0) loss += e * e;
         }
-        g[OB2 + u] = gb2;
-        g[OW3 + u] = gw3;
-        g[OB1 + u] = gb1;
-    }
-    for (int a = 0; a < NH; ++a) {
-        float acc = 0.0f;
-        for (int s2 = 0; s2 < BS; ++s2)
-            acc = fmaf(sd2[s2 * LP + u], sh1[s2 * LP + a], acc);
-        g[OW2 + u * NH + a] = acc;
-    }
+        if (q == 0) gb3 += d3;
+        // d2 (result layout of layer 2 = B operand of the d1 product).
+        bf16x4 d2b[4];
 #pragma unroll
-    for (int k = 0; k < NI; ++k) {
-        float acc = 0.0f;
-        for (int s2 = 0; s2 < BS; ++s2)
-            acc = fmaf(sd1[s2 * LP + u], sxh[s2 * (NI + 1) + k], acc);
-        g[OW1 + u * NI + k] = acc;
+        for (int T = 0; T < 4; ++T)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const float d2 = d3 * W.w3[T][v] * ds2[T][v];
+                d2b[T][v] = (short)bf16_bits(d2);
+                gb2[T][v] += d2;
+                gw3[T][v] = fmaf(d3, h2[T][v], gw3[T][v]);
+            }
+        // d1^T = W2^T d2^T, then times silu'(z1).
+#pragma unroll
+        for (int T = 0; T < 4; ++T) {
+            f32x4 z = zero;
+#pragma unroll
+            for (int tt = 0; tt < 4; ++tt) z = mfma16(w2t[T][tt], d2b[tt], z);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const float d1 = z[v] * ds1[T][v];
+                gb1[T][v] += d1;
+                const int u = 16 * T + 4 * q + v;
+                d1s[r * LDS_ROW + u] = bf16_bits(d1);
+                d2s[r * LDS_ROW + u] = (uint16_t)d2b[T][v];
+                h1s[r * LDS_ROW + u] = (uint16_t)h1b[T][v];
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+            xs[r * 16 + 4 * q + v] = q < 2 ? bf16_bits(xh[v]) : 0;
+        __syncthreads();
+        // Weight gradients: samples on the k dimension.
+        //   gW2[b][a] += sum_s d2[s][b] h1[s][a]:  A[b][s], B[s][a]
+        //   gW1[a][k] += sum_s d1[s][a] xh[s][k]:  A[a][s], B[s][k]
+        bf16x4 ad2[4], ad1[4], bh1[4], bx;
+#pragma unroll
+        for (int T = 0; T < 4; ++T)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int row = (4 * q + j) * LDS_ROW + 16 * T + r;
+                ad2[T][j] = (short)d2s[row];
+                ad1[T][j] = (short)d1s[row];
+                bh1[T][j] = (short)h1s[row];
+            }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) bx[j] = (short)xs[(4 * q + j) * 16 + r];
+#pragma unroll
+        for (int T = 0; T < 4; ++T) {
+#pragma unroll
+            for (int T2 = 0; T2 < 4; ++T2) gw2[T][T2] = mfma16(ad2[T], bh1[T2], gw2[T][T2]);
+            gw1[T] = mfma16(ad1[T], bx, gw1[T]);
+        }
+        __syncthreads();
     }
-    // b3 and the loss: a reduction over the 64 lanes of the wave.
-    float gb3 = sd3[r];
+    // Partial row of this wave.
+    float *g = gpart + (size_t)(blockIdx.x * kWaves + wave) * NP;
+#pragma unroll
+    for (int T = 0; T < 4; ++T)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int u = 16 * T + 4 * q + v;   // row unit of the result tiles
+#pragma unroll
+            for (int T2 = 0; T2 < 4; ++T2) g[OW2 + u * NH + 16 * T2 + r] = gw2[T][T2][v];
+            if (r < NI) g[OW1 + u * NI + r] = gw1[T][v];
+            // Bias / output-layer sums over the 16 samples of this lane group.
+            float a1 = gb1[T][v], a2 = gb2[T][v], a3 = gw3[T][v];
+#pragma unroll
+            for (int o = 1; o < 16; o <<= 1) {
+                a1 += __shfl_xor(a1, o);
+                a2 += __shfl_xor(a2, o);
+                a3 += __shfl_xor(a3, o);
+            }
+            if (r == 0) {
+                g[OB1 + u] = a1;
+                g[OB2 + u] = a2;
+                g[OW3 + u] = a3;
+            }
+        }
     for (int o = 32; o >= 1; o >>= 1) {
         gb3 += __shfl_xor(gb3, o);
         loss += __shfl_xor(loss, o);
     }
-    if (r == 0) {
+    if (l == 0) {
         g[OB3] = gb3;
-        lpart[blockIdx.x] = loss;
+        lpart[blockIdx.x * kWaves + wave] = loss;
     }
 }
 
-// Sum the nblk per-block gradients in block order; with adam != 0 apply one
-// Adam step to the FP32 masters and refresh the BF16 copy.
+// Sum the nrow partial rows in row order; with adam != 0 apply one Adam step
+// to the FP32 masters and refresh the BF16 copy.
 __global__ __launch_bounds__(256) void k_mlp_reduce(
-    int nblk, const float *__restrict__ gpart, float *__restrict__ gsum,
+    int nrow, const float *__restrict__ gpart, float *__restrict__ gsum,
     int adam, float lr, float b1, float b2, float eps, float c1, float c2,
-    float *__restrict__ w, float *__restrict__ wb, float *__restrict__ m,
+    float *__restrict__ w, uint16_t *__restrict__ wh, float *__restrict__ m,
     float *__restrict__ v) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= NP) return;
     float g = 0.0f;
-    for (int b = 0; b < nblk; ++b) g += gpart[(size_t)b * NP + k];
+    for (int b = 0; b < nrow; ++b) g += gpart[(size_t)b * NP + k];
     gsum[k] = g;
     if (adam) {
         const float mk = b1 * m[k] + (1.0f - b1) * g;
@@ -242,13 +363,13 @@ __global__ __launch_bounds__(256) void k_mlp_reduce(
         v[k] = vk;
         const float wk = w[k] - lr * (mk / c1) / (sqrtf(vk / c2) + eps);
         w[k] = wk;
-        wb[k] = bf16r(wk);
+        wh[k] = bf16_bits(wk);
     }
 }
 
-__global__ void k_bf16_copy(const float *__restrict__ w, float *__restrict__ wb) {
+__global__ void k_bf16_copy(const float *__restrict__ w, uint16_t *__restrict__ wh) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < NP) wb[k] = bf16r(w[k]);
+    if (k < NP) wh[k] = bf16_bits(w[k]);
 }
 
 // ---------------------------------------------------------------------------
@@ -258,7 +379,8 @@ __global__ void k_bf16_copy(const float *__restrict__ w, float *__restrict__ wb)
 struct State {
     bool init = false;
     hipStream_t st = nullptr;
-    float *w = nullptr, *wb = nullptr, *m = nullptr, *v = nullptr;
+    float *w = nullptr, *m = nullptr, *v = nullptr;
+    uint16_t *wh = nullptr;      // BF16 copy of the parameters
     float *gsum = nullptr;
     Norm nm{};
     long long adam_t = 0;
@@ -266,7 +388,7 @@ struct State {
     float *dx = nullptr, *dt = nullptr, *dy = nullptr, *gpart = nullptr,
           *lpart = nullptr;
     int *didx = nullptr;
-    size_t cap_n = 0, cap_blk = 0;
+    size_t cap_n = 0, cap_rows = 0;
 };
 State M;
 
@@ -282,7 +404,7 @@ void ensure_init() {
     if (M.init) return;
     MCHK(hipStreamCreateWithFlags(&M.st, hipStreamNonBlocking));
     MCHK(hipMalloc((void **)&M.w, NP * sizeof(float)));
-    MCHK(hipMalloc((void **)&M.wb, NP * sizeof(float)));
+    MCHK(hipMalloc((void **)&M.wh, NP * sizeof(uint16_t)));
     MCHK(hipMalloc((void **)&M.m, NP * sizeof(float)));
     MCHK(hipMalloc((void **)&M.v, NP * sizeof(float)));
     MCHK(hipMalloc((void **)&M.gsum, NP * sizeof(float)));
@@ -295,7 +417,7 @@ void ensure_init() {
     M.init = true;
 }
 
-void ensure_data(size_t n, size_t nblk) {
+void ensure_data(size_t n, size_t nrows) {
     if (n > M.cap_n) {
         if (M.dx) (void)hipFree(M.dx);
         if (M.dt) (void)hipFree(M.dt);
@@ -307,12 +429,12 @@ void ensure_data(size_t n, size_t nblk) {
         MCHK(hipMalloc((void **)&M.didx, n * sizeof(int)));
         M.cap_n = n;
     }
-    if (nblk > M.cap_blk) {
+    if (nrows > M.cap_rows) {
         if (M.gpart) (void)hipFree(M.gpart);
         if (M.lpart) (void)hipFree(M.lpart);
-        MCHK(hipMalloc((void **)&M.gpart, nblk * NP * sizeof(float)));
-        MCHK(hipMalloc((void **)&M.lpart, nblk * sizeof(float)));
-        M.cap_blk = nblk;
+        MCHK(hipMalloc((void **)&M.gpart, nrows * NP * sizeof(float)));
+        MCHK(hipMalloc((void **)&M.lpart, nrows * sizeof(float)));
+        M.cap_rows = nrows;
     }
 }
 
@@ -320,7 +442,7 @@ void set_params(const float *p) {
     ensure_init();
     MCHK(hipMemcpyAsync(M.w, p, NP * sizeof(float), hipMemcpyHostToDevice, M.st));
     hipLaunchKernelGGL(k_bf16_copy, dim3((NP + 255) / 256), dim3(256), 0, M.st,
-                       M.w, M.wb);
+                       M.w, M.wh);
     MCHK(hipMemsetAsync(M.m, 0, NP * sizeof(float), M.st));
     MCHK(hipMemsetAsync(M.v, 0, NP * sizeof(float), M.st));
     MCHK(hipStreamSynchronize(M.st));
@@ -331,11 +453,12 @@ void set_params(const float *p) {
 // idx if given); gradient (mean-square loss) summed into M.gsum, optionally
 // followed by an Adam step.  Returns the summed squared error.
 double grad_step(int count, int base, const int *didx, float lr, int adam) {
-    const int nblk = (count + BS - 1) / BS;
-    ensure_data(M.cap_n, (size_t)nblk);
-    hipLaunchKernelGGL(k_mlp_fwdbwd, dim3(nblk), dim3(BS), 0, M.st, count,
-                       base, didx, M.dx, M.dt, M.wb, M.nm, 1.0f / count,
-                       M.gpart, M.lpart);
+    const int nblk = std::min((count + kGroup - 1) / kGroup, kMaxBlocks);
+    const int nrows = nblk * kWaves;
+    ensure_data(M.cap_n, (size_t)nrows);
+    hipLaunchKernelGGL(k_mlp_fwdbwd, dim3(nblk), dim3(64 * kWaves), 0, M.st,
+                       count, base, didx, M.dx, M.dt, M.wh, M.nm,
+                       1.0f / count, M.gpart, M.lpart);
     float c1 = 1.0f, c2 = 1.0f;
     const float b1 = 0.9f, b2 = 0.999f;
     if (adam) {
@@ -344,11 +467,11 @@ double grad_step(int count, int base, const int *didx, float lr, int adam) {
         c2 = 1.0f - std::pow(b2, (float)M.adam_t);
     }
     hipLaunchKernelGGL(k_mlp_reduce, dim3((NP + 255) / 256), dim3(256), 0,
-                       M.st, nblk, M.gpart, M.gsum, adam, lr, b1, b2, 1e-8f,
-                       c1, c2, M.w, M.wb, M.m, M.v);
+                       M.st, nrows, M.gpart, M.gsum, adam, lr, b1, b2, 1e-8f,
+                       c1, c2, M.w, M.wh, M.m, M.v);
     MCHK(hipGetLastError());
-    std::vector<float> lp(nblk);
-    MCHK(hipMemcpyAsync(lp.data(), M.lpart, nblk * sizeof(float),
+    std::vector<float> lp(nrows);
+    MCHK(hipMemcpyAsync(lp.data(), M.lpart, nrows * sizeof(float),
                         hipMemcpyDeviceToHost, M.st));
     MCHK(hipStreamSynchronize(M.st));
     double l = 0.0;
@@ -424,8 +547,10 @@ MLP_EXPORT int gpunb_mlp_predict(int n, const float *x, float *y) {
     return gpunb::run_guarded("gpunb_mlp_predict", [&] {
         if (n <= 0) return;
         upload_data(n, x, nullptr);
-        hipLaunchKernelGGL(k_mlp_forward, dim3((n + 255) / 256), dim3(256), 0,
-                           M.st, n, M.dx, M.wb, M.nm, M.dy);
+        const int ngroups = (n + 15) / 16;
+        const int nblk = std::min((ngroups + kWaves - 1) / kWaves, 2048);
+        hipLaunchKernelGGL(k_mlp_forward, dim3(nblk), dim3(64 * kWaves), 0,
+                           M.st, n, M.dx, M.wh, M.nm, M.dy);
         MCHK(hipGetLastError());
         MCHK(hipMemcpyAsync(y, M.dy, (size_t)n * sizeof(float),
                             hipMemcpyDeviceToHost, M.st));

@@ -80,6 +80,7 @@ struct Opts {
     std::string log_path;      // training pairs out (--log-train)
     long long log_max = 2000000;
     unsigned log_seed = 12345;
+    double progress = 0.0;     // > 0: a progress line on stderr every so many s
 };
 
 // Features of the learned radius predictor (8 floats per body), all taken
@@ -977,7 +978,7 @@ void parse_args(int argc, char **argv, Opts &o, std::string &in) {
                 "[--nnbopt k] [--lmax k] [--nnbmax k] [--send] [--prec p] "
                 "[--rs0 x] [--rs ac|simple|mlp] [--mlp weights] "
                 "[--nnbmin x] [--log-train file] [--log-max n] "
-                "[--log-seed s]\n",
+                "[--log-seed s] [--progress seconds]\n",
                 argv[0]);
         exit(1);
     }
@@ -1010,6 +1011,7 @@ void parse_args(int argc, char **argv, Opts &o, std::string &in) {
         else if (s == "--log-train") o.log_path = nxt();
         else if (s == "--log-max") o.log_max = atoll(nxt());
         else if (s == "--log-seed") o.log_seed = (unsigned)atol(nxt());
+        else if (s == "--progress") o.progress = atof(nxt());
         else {
             fprintf(stderr, "unknown option %s\n", s.c_str());
             exit(1);
@@ -1085,8 +1087,15 @@ int main(int argc, char **argv) {
            e0, ek, ep);
     fflush(stdout);
     double next = o.dtadj;
+    double last_progress = wtime();
     while (R.t < o.t_end) {
         R.block();
+        if (o.progress > 0.0 && wtime() - last_progress >= o.progress) {
+            last_progress = wtime();
+            fprintf(stderr, "progress t=%.6f blocks=%lld reg_steps=%lld wall=%.1f\n",
+                    R.t, R.blocks, R.reg_steps, last_progress - w0);
+            fflush(stderr);
+        }
         if (R.t >= next) {
             const double e = R.energy(&ek, &ep);
             const double wall = wtime() - w0;

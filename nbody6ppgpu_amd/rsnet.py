@@ -187,17 +187,27 @@ def experiment(n=16384, t_log=0.25, t_run=0.25, seed_train=31, seed_eval=37,
     towards NNBOPT, the same target as the network's, without learning) and
     ``learned``.  The pairs are logged from a ``simple`` run, whose lists
     straddle NNBOPT, so about half of the targets are exact rather than
-    censored.  Returns a dict (fit statistics, the three runs)."""
+    censored.  Files go to ``workdir`` (a temporary directory, removed
+    afterwards, when None).  Returns a dict (fit statistics, the three
+    runs)."""
+    if workdir is None:
+        with tempfile.TemporaryDirectory(prefix="rsnet_") as td:
+            return experiment(n, t_log, t_run, seed_train, seed_eval, log_max,
+                              epochs, nnbopt, etai, etar, timeout, td, log)
     from . import ics, integrate
-    td = workdir or tempfile.mkdtemp(prefix="rsnet_")
+    # Energy checks every dtadj (a power of two no longer than either run).
+    dtadj = 0.125
+    while dtadj > min(t_log, t_run):
+        dtadj *= 0.5
+    td = workdir
     lp = os.path.join(td, "pairs.bin")
     wp = os.path.join(td, "rsnet.bin")
     ct = ics.plummer_segregated(n, seed=seed_train)
     t0 = time.perf_counter()
     integrate.run(ct.m, ct.x, ct.v, t_end=t_log, etai=etai, etar=etar,
-                  nnbopt=nnbopt, timeout=timeout,
+                  dtadj=dtadj, nnbopt=nnbopt, timeout=timeout,
                   extra=("--rs", "simple", "--log-train", lp, "--log-max",
-                         str(log_max)))
+                         str(log_max)), progress=log)
     t_logrun = time.perf_counter() - t0
     F, t, cens = read_log(lp)
     if log:
@@ -210,8 +220,8 @@ def experiment(n=16384, t_log=0.25, t_run=0.25, seed_train=31, seed_eval=37,
     for name, extra in (("ac", ("--rs", "ac")), ("simple", ("--rs", "simple")),
                         ("learned", ("--rs", "mlp", "--mlp", wp))):
         recs = integrate.run(ce.m, ce.x, ce.v, t_end=t_run, etai=etai,
-                             etar=etar, nnbopt=nnbopt, timeout=timeout,
-                             extra=extra)
+                             etar=etar, dtadj=dtadj, nnbopt=nnbopt,
+                             timeout=timeout, extra=extra, progress=log)
         runs[name] = summarise(recs)
         if log:
             log(f"{name}: {runs[name]}")
@@ -220,3 +230,30 @@ def experiment(n=16384, t_log=0.25, t_run=0.25, seed_train=31, seed_eval=37,
     fit_stats["censored_frac"] = float(cens.mean())
     fit_stats["log_run_seconds"] = t_logrun
     return {"n": n, "fit": fit_stats, "runs": runs, "workdir": td}
+
+
+def main(argv=None):
+    """python -m nbody6ppgpu_amd.rsnet [--n N] [--t T] [--epochs E]: the
+    experiment on a mass-segregated Plummer cluster with a Kroupa IMF; prints
+    progress lines, then one JSON object (fit statistics and the three runs)."""
+    import argparse
+    import json
+    import sys
+    ap = argparse.ArgumentParser(prog="python -m nbody6ppgpu_amd.rsnet")
+    ap.add_argument("--n", type=int, default=1048576)
+    ap.add_argument("--t", type=float, default=0.25,
+                    help="N-body time units of the logging and compared runs")
+    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--log-max", type=int, default=2_000_000)
+    a = ap.parse_args(argv)
+    t0 = time.perf_counter()
+    out = experiment(n=a.n, t_log=a.t, t_run=a.t, epochs=a.epochs,
+                     log_max=a.log_max,
+                     log=lambda s: print(f"[{time.perf_counter() - t0:7.1f}s] {s}",
+                                         file=sys.stderr, flush=True))
+    out.pop("workdir")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -3,26 +3,33 @@ INFRASTRUCTURE ONLY: used by tests/ as the checker of the gpunb_mlp_* kernels.
 
 The network has no counterpart in the reference (which sets RS only by the
 Ahmad-Cohen volume rule, src/Main/regcor_gpu.F:146-253); what is restated here
-is the arithmetic the library documents for it:
+is the arithmetic the library documents for it (matrix products on BF16
+operands with FP32 accumulation, biases added in FP32 after the product):
 
-  xh = bf16((x - xmean) / xstd)                     (FP32 ops, then BF16)
-  h1 = bf16(silu(bf16(W1) xh + bf16(b1)))           FP32 accumulation
+  xh = bf16((x - xmean) / xstd)                       (FP32 ops, then BF16)
+  h1 = bf16(silu(bf16(W1) xh + bf16(b1)))
   h2 = bf16(silu(bf16(W2) h1 + bf16(b2)))
   yhat = bf16(W3) h2 + bf16(b3);   y = yhat * ystd + ymean
   L = mean_s (yhat_s - (t_s - ymean) / ystd)^2
 
-bf16() rounds FP32 to the nearest BF16 value, ties to even.  The gradient
-differentiates bf16() as the identity (straight-through).
+bf16() rounds FP32 to the nearest BF16 value, ties to even.  Backward: the
+BF16 rounding is differentiated as the identity (straight-through); the
+deltas d2 = d3 W3 silu'(z2) and d1 = (W2^T bf16(d2)) silu'(z1) enter the
+matrix products (d1 and the weight gradients) as BF16, the bias and
+output-layer gradients use them unrounded.
 
-Two evaluation modes:
-  emulate=True   the FP32 accumulation order of the kernels (each multiply-add
-                 computed exactly in FP64 and rounded once to FP32: the value
-                 of fmaf barring double-rounding ties, probability ~2^-29);
-                 only the kernels' exp() differs (hardware exp2 based), which
-                 can move a bf16() result by one BF16 step on rare elements.
-  emulate=False  plain FP64 with the same BF16 roundings (for the gradient).
-  bf16=False     FP64 without any rounding: a smooth function of the
-                 parameters, used to pin backward() by finite differences.
+Modes:
+  emulate=True   FP32 values as the kernels produce them: every matrix-vector
+                 product is the exact sum of its (exact) BF16 x BF16 products
+                 rounded once to FP32, then the FP32 bias is added.  The
+                 matrix cores may round their partial sums differently (FP32
+                 internal accumulation), and the kernels' exp() is hardware
+                 exp2 based; both can move a bf16() result by one BF16 step on
+                 rare elements, which the tests' tolerances allow for.
+  emulate=False  FP64 with the same BF16 roundings.
+  bf16_round=False  FP64 without any rounding (deltas unrounded too): a
+                 smooth function of the parameters, used to pin backward() by
+                 finite differences.
 
 Parameter vector layout (4801 FP32): W1[64][8] b1[64] W2[64][64] b2[64]
 W3[64] b3.
@@ -56,15 +63,9 @@ def unpack(p):
 
 
 def _matvec_f32(W, b, h):
-    """acc = b; acc = fma(W[:, a], h[:, a], acc) for a in order; FP32."""
-    ns = h.shape[0]
-    acc = np.broadcast_to(b.astype(np.float32), (ns, W.shape[0])).copy()
-    W64 = W.astype(np.float64)
-    h64 = h.astype(np.float64)
-    for a in range(W.shape[1]):
-        acc = (W64[None, :, a] * h64[:, a:a + 1] + acc.astype(np.float64)
-               ).astype(np.float32)
-    return acc
+    """FP32(exact sum_a W[:, a] h[:, a]) + b in FP32 (W, h BF16 values)."""
+    acc = (h.astype(np.float64) @ W.astype(np.float64).T).astype(np.float32)
+    return (acc + b.astype(np.float32)).astype(np.float32)
 
 
 def _silu(z):
@@ -109,23 +110,28 @@ def forward(p, norm, X, emulate=True, bf16_round=True):
 
 def loss_and_grad(p, norm, X, t, emulate=True, bf16_round=True):
     """(L, dL/dp) in FP64 from the forward cache; BF16 weights in the
-    backward pass as in the forward pass."""
+    backward pass as in the forward pass, BF16 deltas in the matrix
+    products (see the module docstring)."""
     xm, xs, ym, ys = norm
     _, (xh, z1, h1, z2, h2, out) = forward(p, norm, X, emulate, bf16_round)
-    r = (lambda a: bf16(a).astype(np.float64)) if bf16_round else (
-        lambda a: np.asarray(a, np.float64))
-    W1, b1, W2, b2, W3, b3 = (r(np.asarray(q, np.float32) if bf16_round else q)
-                              for q in unpack(p))
+    if bf16_round:
+        r = lambda a: bf16(a).astype(np.float64)
+        W1, b1, W2, b2, W3, b3 = (r(np.asarray(q, np.float32)) for q in unpack(p))
+    else:
+        r = lambda a: np.asarray(a, np.float64)
+        W1, b1, W2, b2, W3, b3 = (r(q) for q in unpack(p))
     ns = xh.shape[0]
     e = out - (np.asarray(t, np.float64) - ym) / ys
     L = np.mean(e * e)
     d3 = 2.0 * e / ns                                   # (ns,)
     d2 = d3[:, None] * W3[None, :] * _dsilu(z2)         # (ns, NH)
-    d1 = (d2 @ W2) * _dsilu(z1)                         # (ns, NH)
+    d2q = r(d2)
+    d1 = (d2q @ W2) * _dsilu(z1)                        # (ns, NH)
+    d1q = r(d1)
     g = np.zeros(NP)
-    g[OW1:OB1] = (d1.T @ xh).ravel()
+    g[OW1:OB1] = (d1q.T @ xh).ravel()
     g[OB1:OW2] = d1.sum(0)
-    g[OW2:OB2] = (d2.T @ h1).ravel()
+    g[OW2:OB2] = (d2q.T @ h1).ravel()
     g[OB2:OW3] = d2.sum(0)
     g[OW3:OB3] = d3 @ h2
     g[OB3] = d3.sum()
