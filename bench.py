@@ -205,28 +205,31 @@ def main():
 
     # |dE/E| over one N-body time unit: the host Hermite / Ahmad-Cohen driver
     # (host/hermite_ac.cpp) integrates the same cluster through the engine's
-    # ABI (device predictor + gpunb_regf_ + gpupot_), with the benchmark
-    # configuration's steps (ETAI 0.05, ETAR 0.1) and with the reference's
-    # sample-deck steps (ETAI = ETAR = 0.02).
+    # ABI (device predictor + gpunb_regf_ + gpupot_) with ETAI 0.05 and
+    # ETAR 0.1, once with the mixed-precision pair kernel (FP64 differences,
+    # FP32 pair arithmetic) and once with the all-FP64 kernel, side by side.
     hermite = []
     if not args.no_hermite:
         from nbody6ppgpu_amd import integrate
-        for etai, etar in ((0.05, 0.1), (0.02, 0.02)):
-            log(f"bench: Hermite/AC integration, ETAI {etai} ETAR {etar}")
-            rec = integrate.run(c.m, c.x, c.v, t_end=1.0, etai=etai, etar=etar,
-                                prec=args.prec, timeout=900)
+        for prec, label in ((32, "mixed FP32/FP64"), (64, "FP64")):
+            log(f"bench: Hermite/AC integration, ETAI 0.05 ETAR 0.1, {label}")
+            rec = integrate.run(c.m, c.x, c.v, t_end=1.0, etai=0.05, etar=0.1,
+                                prec=prec, timeout=900)
             last = rec[-1]
             hermite.append({
-                "etai": etai, "etar": etar, "t_end": last["t"],
-                "dE_E": last["dE_E"],
+                "etai": 0.05, "etar": 0.1, "kernel": label, "prec": prec,
+                "t_end": last["t"], "dE_E": last["dE_E"],
                 "max_abs_dE_E": max(abs(r["dE_E"]) for r in rec),
                 "dE_E_checks": [r["dE_E"] for r in rec[1:]],
                 "regular_pairs": last["reg_pairs"],
                 "regular_pairs_per_s": last["reg_pairs"] / max(last["t_reg"], 1e-9),
+                "irregular_pairs": last["irr_pairs"],
+                "interactions_per_s": (last["reg_pairs"] + last["irr_pairs"])
+                                      / max(last["wall"], 1e-9),
                 "irregular_steps": last["irr_steps"],
                 "regular_steps": last["reg_steps"],
                 "blocks": last["blocks"], "nnb_mean": last["nnb_mean"],
-                "wall_s": last["wall"]})
+                "rs_rule": last["rs_rule"], "wall_s": last["wall"]})
 
     # Ensemble: 64 independent Plummer clusters, N log-uniform in
     # [1024, 65536], one regular block step of every body of every cluster,

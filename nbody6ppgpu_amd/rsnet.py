@@ -21,6 +21,7 @@ RS giving NNBOPT neighbours at the next regular step.
 from __future__ import annotations
 
 import ctypes
+import json
 import os
 import tempfile
 import time
@@ -232,12 +233,35 @@ def experiment(n=16384, t_log=0.25, t_run=0.25, seed_train=31, seed_eval=37,
     return {"n": n, "fit": fit_stats, "runs": runs, "workdir": td}
 
 
+def kernel_bench(n_train=2_000_000, n_pred=1_048_576, epochs=3, batch=1024,
+                 seed=5):
+    """Seconds per training epoch and per prediction pass of the network
+    kernels on synthetic features (host arrays, as the driver passes them)."""
+    rng = np.random.default_rng(seed)
+    X = rng.normal(0, 1, (n_train, NI)).astype(np.float32)
+    t = (0.3 * X[:, 7] - 0.1 * X[:, 0] + 0.05 * np.sin(X[:, 2])).astype(np.float32)
+    init(seed)
+    set_norm(X.mean(0), X.std(0), float(t.mean()), float(t.std()))
+    train(X[:batch], t[:batch], 1, batch)              # warm-up
+    t0 = time.perf_counter()
+    loss = train(X, t, epochs, batch, 2e-3, seed)
+    t_train = (time.perf_counter() - t0) / epochs
+    Xp = X[:n_pred]
+    predict(Xp[:1024])
+    t0 = time.perf_counter()
+    predict(Xp)
+    t_pred = time.perf_counter() - t0
+    return {"train_pairs": n_train, "batch": batch, "epochs": epochs,
+            "seconds_per_epoch": t_train, "loss": loss,
+            "predictions": n_pred, "seconds_per_prediction_pass": t_pred,
+            "flops_per_forward": 2 * (NI * NH + NH * NH + NH)}
+
+
 def main(argv=None):
     """python -m nbody6ppgpu_amd.rsnet [--n N] [--t T] [--epochs E]: the
     experiment on a mass-segregated Plummer cluster with a Kroupa IMF; prints
     progress lines, then one JSON object (fit statistics and the three runs)."""
     import argparse
-    import json
     import sys
     ap = argparse.ArgumentParser(prog="python -m nbody6ppgpu_amd.rsnet")
     ap.add_argument("--n", type=int, default=1048576)
@@ -245,7 +269,13 @@ def main(argv=None):
                     help="N-body time units of the logging and compared runs")
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--log-max", type=int, default=2_000_000)
+    ap.add_argument("--kernel-bench", action="store_true",
+                    help="time the network kernels alone on synthetic pairs "
+                         "(2M training pairs, 3 epochs; 1M predictions)")
     a = ap.parse_args(argv)
+    if a.kernel_bench:
+        print(json.dumps(kernel_bench()), flush=True)
+        return
     t0 = time.perf_counter()
     out = experiment(n=a.n, t_log=a.t, t_run=a.t, epochs=a.epochs,
                      log_max=a.log_max,
