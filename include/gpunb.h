@@ -199,6 +199,46 @@ int gpunb_ens_set_bucket_tiles(int nbucket, const int *tiles);
 long long gpunb_ens_pairs(void);
 
 /* ------------------------------------------------------------------------ */
+/* Irregular (neighbour) force: drop-in for the reference's irr_simd library */
+/* (irr.avx.cpp:565-603; callers intgrt.F:175 open, :202-204 and :1270      */
+/* set_jp, :204/:971/:1194 set_list, :545/:680 firr_vec, :1502 close).      */
+/* Addresses and list members are 1-based, as the Fortran host passes them.  */
+/* Pair arithmetic is FP64 (the reference: FP32 on double-single positions). */
+/* ------------------------------------------------------------------------ */
+
+/* Replaces irr_simd_open_ (irr.avx.cpp:567-569): state for nmax particles,
+ * list rows of lmax ints (count + members).  A second open warns. */
+void irr_simd_open_(int *nmax, int *lmax, int *rank);
+/* Replaces irr_simd_close_ (irr.avx.cpp:570-572). */
+void irr_simd_close_(int *rank);
+/* Replaces irr_simd_profile_ (irr.avx.cpp:573-575): calls, <NI>, <NB>, time,
+ * Gflops at 60 flops per pair; counters reset. */
+void irr_simd_profile_(int *rank);
+/* Replaces irr_simd_set_jp_ (irr.avx.cpp:577-587): predictor state of one
+ * particle: position, velocity, acc2 = a/2, jrk6 = jerk/6, mass, time.
+ * Staged on the host, uploaded by the next firr_vec. */
+void irr_simd_set_jp_(int *addr, double pos[3], double vel[3], double acc2[3],
+                      double jrk6[3], double *mass, double *time);
+/* Replaces irr_simd_set_list_ (irr.avx.cpp:588-593): nblist[0] = count,
+ * nblist[1..count] = members (1-based).  Staged like set_jp. */
+void irr_simd_set_list_(int *addr, int *nblist);
+/* Replaces irr_simd_firr_vec_ (irr.avx.cpp:594-602): irregular acceleration
+ * and jerk at time ti of the ni particles addr[] over their lists, every
+ * particle predicted to ti from its own state; nnbid[q] = 1-based nearest
+ * member (smallest r^2, ties to the smaller index), 0 for an empty list. */
+void irr_simd_firr_vec_(double *ti, int *ni, int addr[], double acc[][3],
+                        double jrk[][3], int nnbid[]);
+
+/* Additive, batched forms of set_jp / set_list (no reference counterpart;
+ * same staging).  x, v, f, fd: [n][3]; lists: nnb[q] members of addr[q]
+ * concatenated in `members` (1-based).  Return 0 or -1 (error set). */
+int gpunb_irr_set_jp(int n, const int *addr, const double *x, const double *v,
+                     const double *f, const double *fd, const double *m,
+                     const double *t0);
+int gpunb_irr_set_lists(int n, const int *addr, const int *nnb,
+                        const int *members);
+
+/* ------------------------------------------------------------------------ */
 /* Learned neighbour radius (no reference counterpart: the reference sets RS */
 /* only by the Ahmad-Cohen volume rule, regcor_gpu.F:146-253, which stays    */
 /* the caller's fallback).  An MLP 8 -> 64 -> 64 -> 1 with SiLU activations, */

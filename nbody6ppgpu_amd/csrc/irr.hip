@@ -1,0 +1,499 @@
+// irr.hip -- irregular (neighbour) force on the device: a drop-in for the
+// reference's irr_simd library (irr.avx.cpp:565-603, the Fortran interface
+// irr_simd_open_/close_/profile_/set_jp_/set_list_/firr_vec_), plus batched
+// entry points for callers that hold many particles at once.
+//
+// State in HBM: one 128-byte predictor record per particle (x0, v0, f = a/2,
+// fd = jerk/6, m, t0 in FP64) and a neighbour table of nmax rows of lmax ints
+// (count, then 0-based indices).  set_jp / set_list stage their data on the
+// host; the next firr_vec uploads the staged records and rows in one copy each
+// and scatters them with a kernel, so per-particle Fortran calls cost no
+// device round trip.
+//
+// k_irr_force: one wavefront per active particle.  Every lane predicts its
+// list members with the FP64 polynomial of xbpredall.f:18-26 (the same
+// association as the oracle, no contraction), evaluates the pair in FP64 and
+// accumulates in FP64; the wave then reduces in a fixed order.  The nearest
+// member is the smallest r^2, ties to the smaller index (the reference's
+// packed (r^2, index) minimum, irr.avx.cpp:317-320), returned 1-based, 0 for
+// an empty list.  The reference evaluates the pairs in FP32 on double-single
+// positions; FP64 costs little here (a few tens of pairs per particle).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <ctime>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "gpunb.h"
+#include "gpunb_internal.h"
+
+namespace gpunb_irr {
+
+struct alignas(128) IRec {
+    double x[3], v[3], f[3], fd[3];
+    double m, t0;
+    double pad[2];
+};
+static_assert(sizeof(IRec) == 128, "IRec must be 128 bytes");
+
+#define ICHK(e)                                                               \
+    do {                                                                      \
+        hipError_t r_ = (e);                                                  \
+        if (r_ != hipSuccess)                                                 \
+            throw std::runtime_error(std::string(#e) + ": " +                 \
+                                     hipGetErrorString(r_));                  \
+    } while (0)
+
+#define IREQ(c, msg)                                                          \
+    do {                                                                      \
+        if (!(c)) throw std::runtime_error(msg);                              \
+    } while (0)
+
+// Predicted position and velocity at t (xbpredall.f:18-26 association).
+__device__ inline void predict(const IRec &r, double t, double *x, double *v) {
+    const double s = t - r.t0;
+    const double s1 = 1.5 * s;
+    const double s2 = 2.0 * s;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        x[k] = ((r.fd[k] * s + r.f[k]) * s + r.v[k]) * s + r.x[k];
+        v[k] = (r.fd[k] * s1 + r.f[k]) * s2 + r.v[k];
+    }
+}
+
+constexpr int kWavesPerBlock = 4;
+
+__global__ __launch_bounds__(64 * kWavesPerBlock) void k_irr_force(
+    int ni, const int *__restrict__ idx, double t, const IRec *__restrict__ jp,
+    const int *__restrict__ list, int lstride, double *__restrict__ acc,
+    double *__restrict__ jrk, int *__restrict__ nnbid) {
+    const int lane = threadIdx.x & 63;
+    const int q = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    if (q >= ni) return;                  // uniform per wavefront
+    const int i = idx[q];
+    double xi[3], vi[3];
+    predict(jp[i], t, xi, vi);
+    const int *row = list + (size_t)i * lstride;
+    const int nnb = row[0];
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, j0 = 0.0, j1 = 0.0, j2 = 0.0;
+    double best = INFINITY;
+    int bestj = INT_MAX;
+    for (int k = lane; k < nnb; k += 64) {
+        const int j = row[1 + k];
+        double xj[3], vj[3];
+        predict(jp[j], t, xj, vj);
+        const double dx = xj[0] - xi[0], dy = xj[1] - xi[1], dz = xj[2] - xi[2];
+        const double dvx = vj[0] - vi[0], dvy = vj[1] - vi[1], dvz = vj[2] - vi[2];
+        double r2 = dx * dx;
+        r2 = r2 + dy * dy;
+        r2 = r2 + dz * dz;
+        if (r2 < best || (r2 == best && j < bestj)) {
+            best = r2;
+            bestj = j;
+        }
+        if (r2 == 0.0) continue;
+        const double rinv = 1.0 / sqrt(r2);
+        const double rinv2 = rinv * rinv;
+        const double mr3 = jp[j].m * rinv2 * rinv;
+        double rv = dx * dvx;
+        rv = rv + dy * dvy;
+        rv = rv + dz * dvz;
+        const double alpha = -3.0 * rv * rinv2;
+        a0 += mr3 * dx;
+        a1 += mr3 * dy;
+        a2 += mr3 * dz;
+        j0 += mr3 * (dvx + alpha * dx);
+        j1 += mr3 * (dvy + alpha * dy);
+        j2 += mr3 * (dvz + alpha * dz);
+    }
+    for (int o = 32; o >= 1; o >>= 1) {
+        a0 += __shfl_xor(a0, o);
+        a1 += __shfl_xor(a1, o);
+        a2 += __shfl_xor(a2, o);
+        j0 += __shfl_xor(j0, o);
+        j1 += __shfl_xor(j1, o);
+        j2 += __shfl_xor(j2, o);
+        const double ob = __shfl_xor(best, o);
+        const int oj = __shfl_xor(bestj, o);
+        if (ob < best || (ob == best && oj < bestj)) {
+            best = ob;
+            bestj = oj;
+        }
+    }
+    if (lane == 0) {
+        acc[3 * (size_t)q + 0] = a0;
+        acc[3 * (size_t)q + 1] = a1;
+        acc[3 * (size_t)q + 2] = a2;
+        jrk[3 * (size_t)q + 0] = j0;
+        jrk[3 * (size_t)q + 1] = j1;
+        jrk[3 * (size_t)q + 2] = j2;
+        nnbid[q] = bestj == INT_MAX ? 0 : bestj + 1;
+    }
+}
+
+// Staged predictor records -> their slots.
+__global__ void k_irr_scatter_jp(int n, const int *__restrict__ addr,
+                                 const IRec *__restrict__ src,
+                                 IRec *__restrict__ jp) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) jp[addr[k]] = src[k];
+}
+
+// Staged list rows: hdr[3r] = slot, hdr[3r+1] = offset of the row in flat,
+// hdr[3r+2] = count; one wavefront per row (entries already 0-based).
+__global__ void k_irr_scatter_lists(int nrows, const int *__restrict__ hdr,
+                                    const int *__restrict__ flat,
+                                    int *__restrict__ list, int lstride) {
+    const int lane = threadIdx.x & 63;
+    const int r = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (r >= nrows) return;
+    const int slot = hdr[3 * r], off = hdr[3 * r + 1], cnt = hdr[3 * r + 2];
+    int *row = list + (size_t)slot * lstride;
+    if (lane == 0) row[0] = cnt;
+    for (int k = lane; k < cnt; k += 64) row[1 + k] = flat[off + k];
+}
+
+// Grow-only device / pinned host buffers.
+template <class T>
+struct DBuf {
+    T *p = nullptr;
+    size_t cap = 0;
+    void ensure(size_t n) {
+        if (n <= cap) return;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        ICHK(hipMalloc((void **)&p, std::max<size_t>(n, 1) * sizeof(T)));
+        cap = n;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+template <class T>
+struct HBuf {
+    T *p = nullptr;
+    size_t cap = 0;
+    void ensure(size_t n) {
+        if (n <= cap) return;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        ICHK(hipHostMalloc((void **)&p, std::max<size_t>(n, 1) * sizeof(T)));
+        cap = n;
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+struct State {
+    bool open = false;
+    int nmax = 0, lstride = 0;
+    hipStream_t st = nullptr;
+    DBuf<IRec> jp;
+    DBuf<int> list;
+    // staging (host), latest data per slot wins
+    std::vector<IRec> jp_rec;
+    std::vector<int> jp_addr, jp_slot;        // jp_slot[a] = staged index or -1
+    std::vector<int> ls_hdr, ls_flat, ls_slot;
+    std::vector<int> nnb;                     // host mirror of the counts
+    HBuf<IRec> h_rec;
+    HBuf<int> h_int;
+    DBuf<IRec> d_rec;
+    DBuf<int> d_int;
+    DBuf<double> d_out;
+    DBuf<int> d_nn;
+    HBuf<double> h_out;
+    HBuf<int> h_nn;
+    // profile
+    long long ncall = 0, nsteps = 0, ninter = 0;
+    double t_force = 0.0;
+};
+State I;
+
+double wtime() {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+bool quiet() {
+    const char *q = getenv("GPUNB_QUIET");
+    return q && *q && *q != '0';
+}
+
+void do_open(int nmax, int lmax, int rank) {
+    if (I.open) {
+        fprintf(stderr, "irr_simd: it is already open\n");
+        return;
+    }
+    IREQ(nmax > 0, "irr_simd_open: nmax must be > 0");
+    IREQ(lmax >= 1, "irr_simd_open: lmax must be >= 1");
+    ICHK(hipStreamCreateWithFlags(&I.st, hipStreamNonBlocking));
+    I.nmax = nmax;
+    I.lstride = lmax;
+    I.jp.ensure((size_t)nmax);
+    I.list.ensure((size_t)nmax * lmax);
+    // Empty lists and a parked predictor state until the caller sets them.
+    ICHK(hipMemsetAsync(I.list.p, 0, (size_t)nmax * lmax * sizeof(int), I.st));
+    ICHK(hipMemsetAsync(I.jp.p, 0, (size_t)nmax * sizeof(IRec), I.st));
+    ICHK(hipStreamSynchronize(I.st));
+    I.jp_slot.assign(nmax, -1);
+    I.ls_slot.assign(nmax, -1);
+    I.nnb.assign(nmax, 0);
+    I.jp_rec.clear();
+    I.jp_addr.clear();
+    I.ls_hdr.clear();
+    I.ls_flat.clear();
+    I.ncall = I.nsteps = I.ninter = 0;
+    I.t_force = 0.0;
+    I.open = true;
+    if (!quiet())
+        fprintf(stderr, "# Opening irr_simd lib. MI355X ver. - rank: %d; nmax: %d, lmax: %d\n",
+                rank, nmax, lmax);
+}
+
+void do_close(int rank) {
+    if (!I.open) {
+        fprintf(stderr, "irr_simd: it is already close\n");
+        return;
+    }
+    ICHK(hipStreamSynchronize(I.st));
+    I.jp.release();
+    I.list.release();
+    I.d_rec.release();
+    I.d_int.release();
+    I.d_out.release();
+    I.d_nn.release();
+    I.h_rec.release();
+    I.h_int.release();
+    I.h_out.release();
+    I.h_nn.release();
+    (void)hipStreamDestroy(I.st);
+    I.st = nullptr;
+    I.open = false;
+    if (!quiet()) fprintf(stderr, "Closing irr_simd lib. MI355X ver. - rank: %d\n", rank);
+}
+
+void stage_jp(int a, const double *x, const double *v, const double *f,
+              const double *fd, double m, double t0) {
+    IREQ(a >= 1 && a <= I.nmax, "address out of range [1, nmax]");
+    IRec r{};
+    for (int k = 0; k < 3; ++k) {
+        r.x[k] = x[k];
+        r.v[k] = v[k];
+        r.f[k] = f[k];
+        r.fd[k] = fd[k];
+    }
+    r.m = m;
+    r.t0 = t0;
+    const int s = a - 1;
+    if (I.jp_slot[s] >= 0) {
+        I.jp_rec[I.jp_slot[s]] = r;
+    } else {
+        I.jp_slot[s] = (int)I.jp_rec.size();
+        I.jp_rec.push_back(r);
+        I.jp_addr.push_back(s);
+    }
+}
+
+// nb: 1-based member indices.
+void stage_list(int a, int cnt, const int *nb) {
+    IREQ(a >= 1 && a <= I.nmax, "address out of range [1, nmax]");
+    IREQ(cnt >= 0 && cnt <= I.lstride - 1, "list longer than lmax - 1");
+    const int s = a - 1;
+    if (I.ls_slot[s] >= 0) I.ls_hdr[3 * I.ls_slot[s]] = -1;   // superseded
+    I.ls_slot[s] = (int)(I.ls_hdr.size() / 3);
+    I.ls_hdr.push_back(s);
+    I.ls_hdr.push_back((int)I.ls_flat.size());
+    I.ls_hdr.push_back(cnt);
+    for (int k = 0; k < cnt; ++k) {
+        const int j = nb[k];
+        IREQ(j >= 1 && j <= I.nmax, "list member out of range [1, nmax]");
+        I.ls_flat.push_back(j - 1);
+    }
+    I.nnb[s] = cnt;
+}
+
+// Upload the staged records and list rows, scatter them, then (ni > 0) the
+// force of addr[] at t; one host synchronisation at the end.  Layout of the
+// pinned int buffer: [jp slots (nj)] [row headers (3 nr)] [members] [i (ni)].
+void flush_and_force(double t, int ni, const int *addr, double *acc,
+                     double *jrk, int *nnbid) {
+    const size_t nj = I.jp_rec.size();
+    std::vector<int> hdr;
+    hdr.reserve(I.ls_hdr.size());
+    for (size_t r = 0; r < I.ls_hdr.size(); r += 3)
+        if (I.ls_hdr[r] >= 0)
+            hdr.insert(hdr.end(), I.ls_hdr.begin() + r, I.ls_hdr.begin() + r + 3);
+    const size_t nr = hdr.size() / 3, nf = I.ls_flat.size();
+    const size_t o_hdr = nj, o_flat = o_hdr + 3 * nr, o_idx = o_flat + nf;
+    const size_t nint = o_idx + (size_t)ni;
+    long long pairs = 0;
+    if (nint) {
+        I.h_int.ensure(nint);
+        I.d_int.ensure(nint);
+        int *h = I.h_int.p;
+        memcpy(h, I.jp_addr.data(), nj * sizeof(int));
+        memcpy(h + o_hdr, hdr.data(), 3 * nr * sizeof(int));
+        memcpy(h + o_flat, I.ls_flat.data(), nf * sizeof(int));
+        for (int q = 0; q < ni; ++q) {
+            const int a = addr[q];
+            IREQ(a >= 1 && a <= I.nmax, "address out of range [1, nmax]");
+            h[o_idx + q] = a - 1;
+        }
+        ICHK(hipMemcpyAsync(I.d_int.p, h, nint * sizeof(int),
+                            hipMemcpyHostToDevice, I.st));
+    }
+    if (nj) {
+        I.h_rec.ensure(nj);
+        I.d_rec.ensure(nj);
+        memcpy(I.h_rec.p, I.jp_rec.data(), nj * sizeof(IRec));
+        ICHK(hipMemcpyAsync(I.d_rec.p, I.h_rec.p, nj * sizeof(IRec),
+                            hipMemcpyHostToDevice, I.st));
+        hipLaunchKernelGGL(k_irr_scatter_jp, dim3((unsigned)((nj + 255) / 256)),
+                           dim3(256), 0, I.st, (int)nj, I.d_int.p, I.d_rec.p,
+                           I.jp.p);
+    }
+    if (nr)
+        hipLaunchKernelGGL(k_irr_scatter_lists, dim3((unsigned)((nr + 3) / 4)),
+                           dim3(256), 0, I.st, (int)nr, I.d_int.p + o_hdr,
+                           I.d_int.p + o_flat, I.list.p, I.lstride);
+    for (int s : I.jp_addr) I.jp_slot[s] = -1;
+    for (size_t r = 0; r < I.ls_hdr.size(); r += 3)
+        if (I.ls_hdr[r] >= 0) I.ls_slot[I.ls_hdr[r]] = -1;
+    // Counts of the members sent now are already in the host mirror.
+    for (int q = 0; q < ni; ++q) pairs += I.nnb[addr[q] - 1];
+    I.jp_rec.clear();
+    I.jp_addr.clear();
+    I.ls_hdr.clear();
+    I.ls_flat.clear();
+    if (ni > 0) {
+        I.d_out.ensure((size_t)ni * 6);
+        I.d_nn.ensure((size_t)ni);
+        I.h_out.ensure((size_t)ni * 6);
+        I.h_nn.ensure((size_t)ni);
+        hipLaunchKernelGGL(k_irr_force,
+                           dim3((unsigned)((ni + kWavesPerBlock - 1) / kWavesPerBlock)),
+                           dim3(64 * kWavesPerBlock), 0, I.st, ni,
+                           I.d_int.p + o_idx, t, I.jp.p, I.list.p, I.lstride,
+                           I.d_out.p, I.d_out.p + 3 * (size_t)ni, I.d_nn.p);
+        ICHK(hipMemcpyAsync(I.h_out.p, I.d_out.p, (size_t)ni * 6 * sizeof(double),
+                            hipMemcpyDeviceToHost, I.st));
+        ICHK(hipMemcpyAsync(I.h_nn.p, I.d_nn.p, (size_t)ni * sizeof(int),
+                            hipMemcpyDeviceToHost, I.st));
+    }
+    ICHK(hipGetLastError());
+    // The pinned staging buffers are reused by the next call.
+    if (nint || nj) ICHK(hipStreamSynchronize(I.st));
+    if (ni > 0) {
+        memcpy(acc, I.h_out.p, (size_t)ni * 3 * sizeof(double));
+        memcpy(jrk, I.h_out.p + 3 * (size_t)ni, (size_t)ni * 3 * sizeof(double));
+        if (nnbid) memcpy(nnbid, I.h_nn.p, (size_t)ni * sizeof(int));
+        I.ncall++;
+        I.nsteps += ni;
+        I.ninter += pairs;
+    }
+}
+
+void firr(double t, int ni, const int *addr, double *acc, double *jrk,
+          int *nnbid) {
+    IREQ(I.open, "irr_simd is not open");
+    IREQ(ni >= 0, "ni must be >= 0");
+    const double w0 = wtime();
+    flush_and_force(t, ni, addr, acc, jrk, nnbid);
+    I.t_force += wtime() - w0;
+}
+
+void profile(int rank) {
+    if (!I.open || quiet()) return;
+    const double gf = I.t_force > 0 ? 60.0 * (double)I.ninter * 1e-9 / I.t_force : 0.0;
+    fprintf(stderr,
+            "[R.%d MI355X Irr.F ] Ncall: %lld <NI>: %lld <NB>: %f grav: %f s, %f Gflops, %f usec\n",
+            rank, I.ncall, I.ncall ? I.nsteps / I.ncall : 0,
+            I.nsteps ? (double)I.ninter / I.nsteps : 0.0, I.t_force, gf,
+            I.ncall ? 1e6 * I.t_force / I.ncall : 0.0);
+    I.ncall = I.nsteps = I.ninter = 0;
+    I.t_force = 0.0;
+}
+
+}  // namespace gpunb_irr
+
+using namespace gpunb_irr;
+
+#define IRR_EXPORT __attribute__((visibility("default")))
+
+extern "C" {
+
+IRR_EXPORT void irr_simd_open_(int *nmax, int *lmax, int *rank) {
+    gpunb::run_guarded("irr_simd_open_", [&] { do_open(*nmax, *lmax, *rank); });
+}
+
+IRR_EXPORT void irr_simd_close_(int *rank) {
+    gpunb::run_guarded("irr_simd_close_", [&] { do_close(*rank); });
+}
+
+IRR_EXPORT void irr_simd_profile_(int *rank) {
+    gpunb::run_guarded("irr_simd_profile_", [&] { profile(*rank); });
+}
+
+IRR_EXPORT void irr_simd_set_jp_(int *addr, double pos[3], double vel[3],
+                                 double acc2[3], double jrk6[3], double *mass,
+                                 double *time) {
+    gpunb::run_guarded("irr_simd_set_jp_", [&] {
+        IREQ(I.open, "irr_simd is not open");
+        stage_jp(*addr, pos, vel, acc2, jrk6, *mass, *time);
+    });
+}
+
+IRR_EXPORT void irr_simd_set_list_(int *addr, int *nblist) {
+    gpunb::run_guarded("irr_simd_set_list_", [&] {
+        IREQ(I.open, "irr_simd is not open");
+        stage_list(*addr, nblist[0], nblist + 1);
+    });
+}
+
+IRR_EXPORT void irr_simd_firr_vec_(double *ti, int *ni, int addr[],
+                                   double acc[][3], double jrk[][3],
+                                   int nnbid[]) {
+    gpunb::run_guarded("irr_simd_firr_vec_", [&] {
+        firr(*ti, *ni, addr, &acc[0][0], &jrk[0][0], nnbid);
+    });
+}
+
+IRR_EXPORT int gpunb_irr_set_jp(int n, const int *addr, const double *x,
+                                const double *v, const double *f,
+                                const double *fd, const double *m,
+                                const double *t0) {
+    return gpunb::run_guarded("gpunb_irr_set_jp", [&] {
+        IREQ(I.open, "irr_simd is not open");
+        for (int q = 0; q < n; ++q)
+            stage_jp(addr[q], x + 3 * (size_t)q, v + 3 * (size_t)q,
+                     f + 3 * (size_t)q, fd + 3 * (size_t)q, m[q], t0[q]);
+    });
+}
+
+IRR_EXPORT int gpunb_irr_set_lists(int n, const int *addr, const int *nnb,
+                                   const int *members) {
+    return gpunb::run_guarded("gpunb_irr_set_lists", [&] {
+        IREQ(I.open, "irr_simd is not open");
+        size_t off = 0;
+        for (int q = 0; q < n; ++q) {
+            stage_list(addr[q], nnb[q], members + off);
+            off += (size_t)nnb[q];
+        }
+    });
+}
+
+}  // extern "C"

@@ -248,7 +248,8 @@ __global__ void k_zero_int(int *p, int n) {
 // scalar cache as SGPR operands: no LDS and no VGPRs are spent on them.
 //
 // Per pair (fast path): FP64 differences dx = x_j - x_i, dv = v_j - v_i
-// (exactly the oracle's), rounded to FP32; FP32 r2, r2p, dx.dv, one v_rsq_f32
+// (exactly the oracle's), rounded to FP32; FP32 r2, dx.dv, |dv|^2, the
+// predicted distance r2p = r2 + 2 dtr dx.dv + dtr^2 |dv|^2, one v_rsq_f32
 // and the force sums.  The neighbour test first compares the FP32
 // min(r2, r2p) with a per-lane threshold that is provably above the FP64
 // threshold by more than the FP32 evaluation error; only lanes below it take
@@ -297,7 +298,7 @@ __global__ __launch_bounds__(64) void k_regf_pass1(
 
     double x0 = 0.0, y0 = 0.0, z0 = 0.0, u0 = 0.0, w0 = 0.0, q0 = 0.0;
     double h2d = 0.0, dtrd = 0.0;
-    float dtr32 = 0.0f;
+    float dtr2x = 0.0f, dtrsq = 0.0f;   // 2 dtr and dtr^2 (FP32)
     float thr = -1.0f;  // inactive lanes never enter the FP64 path
     if (act) {
         x0 = xi[3 * (size_t)i + 0];
@@ -310,12 +311,21 @@ __global__ __launch_bounds__(64) void k_regf_pass1(
         dtrd = dtr[i];
         const double chk = x0 + y0 + z0 + u0 + w0 + q0 + h2d + dtrd;
         if (chk != chk) atomicOr(err, ERR_NAN_I);
-        dtr32 = (float)dtrd;
-        // FP32 screening threshold.  With rho = |dtr| (|v_i| + max|v_j|) /
-        // sqrt(H), H the FP64 threshold (h2, or m_j h2 >= m_min h2), the FP32
-        // values of r2 and r2p exceed their FP64 values by at most a relative
-        // 5.1u and by (1 + 4u(1 + 2 rho))^2 (1 + 3u) - 1 respectively whenever
-        // the FP64 value is below H.  The threshold doubles that margin.
+        const float dtr32 = (float)dtrd;
+        dtr2x = 2.0f * dtr32;
+        dtrsq = dtr32 * dtr32;
+        // FP32 screening threshold.  The fast path evaluates
+        //   r2p = fma(dtr^2, |dv|^2, fma(2 dtr, dx.dv, r2))
+        // from the FP32-rounded differences.  Every operand and operation
+        // contributes a relative error of at most u to one of the three
+        // nonnegative magnitudes r2, 2|dtr| sum|dx_c dv_c|, dtr^2 |dv|^2, so
+        // |r2p_32 - r2p| <= 10u S with S = sum_c (|dx_c| + |dtr| |dv_c|)^2
+        // <= (|dx| + |dtr| |dv|)^2.  If r2p < H (H the FP64 threshold: h2, or
+        // m_j h2 >= m_min h2), then |dx| <= sqrt(H) + |dtr| |dv| and, with
+        // rho = |dtr| (|v_i| + max|v_j|) / sqrt(H), S <= H (1 + 2 rho)^2.
+        // r2 itself is within a relative 5.1u.  The threshold doubles the
+        // r2p margin and adds 16u (24u with the mass factor) for the
+        // rounding of the FP64 reference values and of the threshold.
         if (h2d > 0.0) {
             const double vmax = bitsd(jst->vmax_bits);
             double hmin = h2d;
@@ -326,9 +336,8 @@ __global__ __launch_bounds__(64) void k_regf_pass1(
             const double u = (double)kU32;
             const double rho =
                 fabs(dtrd) * (speed_up(u0, w0, q0) + vmax) / sqrt(hmin);
-            const double g = 1.0 + 4.0 * u * (1.0 + 2.0 * rho);
-            const double eps = 2.0 * (g * g * (1.0 + 3.0 * u) - 1.0) +
-                               (MFLAG ? 24.0 : 16.0) * u;
+            const double g = 1.0 + 2.0 * rho;
+            const double eps = 2.0 * 10.0 * u * g * g + (MFLAG ? 24.0 : 16.0) * u;
             const double tt = h2d * (1.0 + eps);
             float t32 = (float)tt;
             if ((double)t32 < tt) t32 = nextafterf(t32, INFINITY);
@@ -346,8 +355,8 @@ __global__ __launch_bounds__(64) void k_regf_pass1(
     // FP32 force terms of one pair, shared by the screened fast path and the
     // FP64 resolution of band pairs.
     auto pair_terms = [&](const JRec &J, float fx, float fy, float fz,
-                          float gx, float gy, float gz, float r2, float rinv) {
-        const float rv = fmaf(fz, gz, fmaf(fy, gy, fx * gx));
+                          float gx, float gy, float gz, float r2, float rv,
+                          float rinv) {
         const float rinv2 = rinv * rinv;
         const float mr = J.m32 * rinv;
         const float mr3 = mr * rinv2;
@@ -380,10 +389,12 @@ __global__ __launch_bounds__(64) void k_regf_pass1(
             ++c;
         } else {
             const float fx = (float)dx, fy = (float)dy, fz = (float)dz;
+            const float gx = (float)dvx, gy = (float)dvy, gz = (float)dvz;
             const float r2 = fmaf(fz, fz, fmaf(fy, fy, fx * fx));
+            const float rv = fmaf(fz, gz, fmaf(fy, gy, fx * gx));
             if (r2 != 0.0f)
-                pair_terms(J, fx, fy, fz, (float)dvx, (float)dvy, (float)dvz,
-                           r2, __builtin_amdgcn_rsqf(r2));
+                pair_terms(J, fx, fy, fz, gx, gy, gz, r2, rv,
+                           __builtin_amdgcn_rsqf(r2));
         }
     };
 
@@ -405,16 +416,15 @@ __global__ __launch_bounds__(64) void k_regf_pass1(
                 const float gy = (float)(J[k].vy - w0);
                 const float gz = (float)(J[k].vz - q0);
                 const float r2 = fmaf(fz, fz, fmaf(fy, fy, fx * fx));
-                const float px = fmaf(dtr32, gx, fx);
-                const float py = fmaf(dtr32, gy, fy);
-                const float pz = fmaf(dtr32, gz, fz);
-                const float r2p = fmaf(pz, pz, fmaf(py, py, px * px));
+                const float rv = fmaf(fz, gz, fmaf(fy, gy, fx * gx));
+                const float v2 = fmaf(gz, gz, fmaf(gy, gy, gx * gx));
+                const float r2p = fmaf(dtrsq, v2, fmaf(dtr2x, rv, r2));
                 float tj = thr;
                 if (MFLAG) tj = fmaxf(thr * J[k].m32, FLT_MIN);
                 fl[k] = fminf(r2, r2p) < tj;
                 any = any || fl[k];
                 const float rinv = fl[k] ? 0.0f : __builtin_amdgcn_rsqf(r2);
-                pair_terms(J[k], fx, fy, fz, gx, gy, gz, r2, rinv);
+                pair_terms(J[k], fx, fy, fz, gx, gy, gz, r2, rv, rinv);
             }
             if (any) {
 #pragma unroll

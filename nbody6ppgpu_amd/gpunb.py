@@ -108,6 +108,15 @@ def _declare(lib: ctypes.CDLL) -> None:
                             [ctypes.c_int, _fp, _fp, ctypes.c_int,
                              ctypes.c_int, ctypes.c_float, ctypes.c_uint,
                              ctypes.POINTER(ctypes.c_double)]),
+        "irr_simd_open_": (v, [_ip, _ip, _ip]),
+        "irr_simd_close_": (v, [_ip]),
+        "irr_simd_profile_": (v, [_ip]),
+        "irr_simd_set_jp_": (v, [_ip, _dp, _dp, _dp, _dp, _dp, _dp]),
+        "irr_simd_set_list_": (v, [_ip, _ip]),
+        "irr_simd_firr_vec_": (v, [_dp, _ip, _ip, _dp, _dp, _ip]),
+        "gpunb_irr_set_jp": (ctypes.c_int, [ctypes.c_int, _ip, _dp, _dp, _dp,
+                                            _dp, _dp, _dp]),
+        "gpunb_irr_set_lists": (ctypes.c_int, [ctypes.c_int, _ip, _ip, _ip]),
         "gpunb_timing": (ctypes.c_int, [ctypes.c_int]),
         "gpunb_timing_read": (ctypes.c_int,
                               [ctypes.POINTER(ctypes.c_double),
@@ -308,3 +317,58 @@ def gpunb_stream() -> int:
 
 def version() -> str:
     return load().gpunb_version().decode()
+
+
+# --- irregular force (irr_simd_* drop-in; 1-based addresses) ---------------
+
+def irr_open(nmax: int, lmax: int, rank: int = 0) -> None:
+    load().irr_simd_open_(_i(nmax), _i(lmax), _i(rank))
+    _check()
+
+
+def irr_close(rank: int = 0) -> None:
+    load().irr_simd_close_(_i(rank))
+    _check()
+
+
+def irr_profile(rank: int = 0) -> None:
+    load().irr_simd_profile_(_i(rank))
+    _check()
+
+
+def irr_set_jp(addr, x0, v0, f, fd, m, t0) -> None:
+    """Predictor state of particles addr (1-based): x0, v0, f = a/2,
+    fd = jerk/6 ([n][3]), mass and t0 ([n])."""
+    a = np.ascontiguousarray(addr, dtype=np.int32)
+    n = a.shape[0]
+    arrs = [_f64(q, (n, 3)) for q in (x0, v0, f, fd)]
+    if load().gpunb_irr_set_jp(n, _p(a), *[_p(q) for q in arrs],
+                               _p(_f64(m, (n,))), _p(_f64(t0, (n,)))) != 0:
+        _check()
+
+
+def irr_set_lists(addr, lists) -> None:
+    """lists[q]: 1-based members of particle addr[q]."""
+    a = np.ascontiguousarray(addr, dtype=np.int32)
+    nnb = np.array([len(l) for l in lists], dtype=np.int32)
+    flat = np.ascontiguousarray(
+        np.concatenate([np.asarray(l, dtype=np.int32) for l in lists])
+        if len(lists) else np.zeros(0, np.int32), dtype=np.int32)
+    if flat.size == 0:
+        flat = np.zeros(1, np.int32)
+    if load().gpunb_irr_set_lists(a.shape[0], _p(a), _p(nnb), _p(flat)) != 0:
+        _check()
+
+
+def irr_firr(t: float, addr):
+    """(acc [ni][3], jrk [ni][3], nnbid [ni] 1-based) at time t."""
+    a = np.ascontiguousarray(addr, dtype=np.int32)
+    ni = a.shape[0]
+    acc = np.zeros((max(ni, 1), 3))
+    jrk = np.zeros((max(ni, 1), 3))
+    nn = np.zeros(max(ni, 1), dtype=np.int32)
+    tt = ctypes.c_double(float(t))
+    load().irr_simd_firr_vec_(ctypes.byref(tt), _i(ni), _p(a if ni else nn),
+                              _p(acc), _p(jrk), _p(nn))
+    _check()
+    return acc[:ni], jrk[:ni], nn[:ni]

@@ -12,7 +12,8 @@
 // evolution, no tidal field.
 //
 // Per particle i the force is split in an irregular part (neighbours in the
-// list L_i, summed here in FP64 every irregular step dt_i) and a regular part
+// list L_i, summed in FP64 every irregular step dt_i, on the device through
+// the irr_simd_* entry points or, with --irr host, on the host) and a regular part
 // (all other particles, from gpunb_regf_ every regular step dtr_i, extended
 // between regular steps by its Taylor polynomial).  Hermite interpolation of
 // two force/jerk pairs gives the second and third derivatives for the
@@ -81,6 +82,8 @@ struct Opts {
     long long log_max = 2000000;
     unsigned log_seed = 12345;
     double progress = 0.0;     // > 0: a progress line on stderr every so many s
+    int irr_dev = 1;           // irregular force: 1 device (irr_simd_*), 0 host
+    int irr_min = 256;         // blocks with fewer bodies: host irregular force
 };
 
 // Features of the learned radius predictor (8 floats per body), all taken
@@ -210,7 +213,8 @@ struct Run {
     // work counters
     long long reg_pairs = 0, irr_pairs = 0, reg_calls = 0, irr_steps = 0,
               reg_steps = 0, blocks = 0, overflows = 0;
-    double t_reg = 0.0, t_irr = 0.0, t_pred = 0.0, t_pot = 0.0, t_mlp = 0.0;
+    double t_reg = 0.0, t_irr = 0.0, t_pred = 0.0, t_pot = 0.0, t_mlp = 0.0,
+           t_jp = 0.0;
     // Block-step schedule: next time -> bodies due then (times are dyadic).
     std::map<double, std::vector<int>> due;
     // staging for the ABI
@@ -267,7 +271,8 @@ struct Run {
     }
 
     void jpset(const std::vector<int> &idx) {
-        if (!o.use_predict || idx.empty()) return;
+        if ((!o.use_predict && !o.irr_dev) || idx.empty()) return;
+        const double w0 = wtime();
         const int k = (int)idx.size();
         std::vector<int> addr(k);
         std::vector<double> X(3 * k), V(3 * k), F(3 * k), FD(3 * k), M(k), T(k);
@@ -284,9 +289,43 @@ struct Run {
             T[q] = b.t0;
         }
         int kk = k;
+        if (o.use_predict)
         gpunb_jpset_(&kk, addr.data(), (double(*)[3])X.data(),
                      (double(*)[3])V.data(), (double(*)[3])F.data(),
                      (double(*)[3])FD.data(), M.data(), T.data());
+        abi_check();
+        if (o.irr_dev) {
+            gpunb_irr_set_jp(k, addr.data(), X.data(), V.data(), F.data(),
+                             FD.data(), M.data(), T.data());
+            abi_check();
+        }
+        t_jp += wtime() - w0;
+    }
+
+    // Lists of bodies idx (0-based members) to the irregular library.
+    void irr_lists(const std::vector<int> &idx,
+                   const std::vector<std::vector<int>> &L) {
+        if (!o.irr_dev || idx.empty()) return;
+        std::vector<int> addr(idx.size()), cnt(idx.size()), flat;
+        for (size_t q = 0; q < idx.size(); ++q) {
+            addr[q] = idx[q] + 1;
+            cnt[q] = (int)L[q].size();
+            for (int j : L[q]) flat.push_back(j + 1);
+        }
+        if (flat.empty()) flat.push_back(0);
+        gpunb_irr_set_lists((int)idx.size(), addr.data(), cnt.data(), flat.data());
+        abi_check();
+    }
+
+    // Irregular force at t of bodies idx on the device (their lists as last
+    // sent): F, FD [idx.size()][3].
+    void irr_dev(double t, const std::vector<int> &idx, double *F, double *FD) {
+        std::vector<int> addr(idx.size()), nn(idx.size());
+        for (size_t q = 0; q < idx.size(); ++q) addr[q] = idx[q] + 1;
+        int ni = (int)idx.size();
+        double tt = t;
+        irr_simd_firr_vec_(&tt, &ni, addr.data(), (double(*)[3])F,
+                           (double(*)[3])FD, nn.data());
         abi_check();
     }
 
@@ -635,11 +674,25 @@ struct Run {
         std::vector<std::vector<int>> L;
         regular(R, X, V, FR, FRD, L);
         double w0 = wtime();
+        std::vector<double> FI0, FID0;
+        if (o.irr_dev) {
+            irr_lists(all, L);
+            FI0.resize(3 * (size_t)n);
+            FID0.resize(3 * (size_t)n);
+            irr_dev(0.0, all, FI0.data(), FID0.data());
+        }
 #pragma omp parallel for schedule(dynamic, 64)
         for (int q = 0; q < n; ++q) {
             Body &b = B[q];
             b.list = L[q];
-            irr_force(B, b.list, 0.0, b.x0, b.v0, b.fi, b.fid);
+            if (o.irr_dev) {
+                for (int c = 0; c < 3; ++c) {
+                    b.fi[c] = FI0[3 * (size_t)q + c];
+                    b.fid[c] = FID0[3 * (size_t)q + c];
+                }
+            } else {
+                irr_force(B, b.list, 0.0, b.x0, b.v0, b.fi, b.fid);
+            }
             for (int c = 0; c < 3; ++c) {
                 b.fr[c] = FR[3 * (size_t)q + c];
                 b.frd[c] = FRD[3 * (size_t)q + c];
@@ -710,10 +763,18 @@ struct Run {
         // Irregular force with the current lists.
         std::vector<double> FI(3 * (size_t)na), FID(3 * (size_t)na);
         double w0 = wtime();
-#pragma omp parallel for schedule(dynamic, 16)
-        for (int k = 0; k < na; ++k)
-            irr_force(B, B[A[k]].list, tn, &XP[3 * (size_t)k], &VP[3 * (size_t)k],
-                      &FI[3 * (size_t)k], &FID[3 * (size_t)k]);
+        // Device for large blocks; small ones stay on the host, where a call
+        // costs less than a device round trip (the device state is staged
+        // either way).
+        if (o.irr_dev && na >= o.irr_min) {
+            irr_dev(tn, A, FI.data(), FID.data());
+        } else {
+#pragma omp parallel for schedule(dynamic, 16) if (na >= 64)
+            for (int k = 0; k < na; ++k)
+                irr_force(B, B[A[k]].list, tn, &XP[3 * (size_t)k],
+                          &VP[3 * (size_t)k], &FI[3 * (size_t)k],
+                          &FID[3 * (size_t)k]);
+        }
         t_irr += wtime() - w0;
         for (int k = 0; k < na; ++k) irr_pairs += (long long)B[A[k]].list.size();
         irr_steps += na;
@@ -753,12 +814,17 @@ struct Run {
         std::vector<int> TC(R.size(), -1);
         const bool logging = !o.log_path.empty();
         w0 = wtime();
-#pragma omp parallel for schedule(dynamic, 16)
-        for (int q = 0; q < (int)R.size(); ++q) {
+        const int nr = (int)R.size();
+        const bool dev_r = o.irr_dev && nr >= o.irr_min;
+        if (o.irr_dev && nr > 0) irr_lists(R, LN);
+        if (dev_r) irr_dev(tn, R, FIL.data(), FIDL.data());
+#pragma omp parallel for schedule(dynamic, 16) if (nr >= 64)
+        for (int q = 0; q < nr; ++q) {
             const int a = rpos[R[q]];
             const double *xi = &XP[3 * (size_t)a], *vi = &VP[3 * (size_t)a];
-            irr_force(B, LN[q], tn, xi, vi, &FIL[3 * (size_t)q],
-                      &FIDL[3 * (size_t)q]);
+            if (!dev_r)
+                irr_force(B, LN[q], tn, xi, vi, &FIL[3 * (size_t)q],
+                          &FIDL[3 * (size_t)q]);
             const Body &bi = B[R[q]];
             if (LN[q].size() <= 3) {
                 for (int jb : LN[q]) {
@@ -822,7 +888,7 @@ struct Run {
         std::vector<char> WANT(R.size(), 0), FELL(R.size(), 0);
 
 
-#pragma omp parallel for schedule(dynamic, 16)
+#pragma omp parallel for schedule(dynamic, 16) if (na >= 64)
         for (int k = 0; k < na; ++k) {
             Body &b = B[A[k]];
             const double h = tn - b.t0;
@@ -978,7 +1044,8 @@ void parse_args(int argc, char **argv, Opts &o, std::string &in) {
                 "[--nnbopt k] [--lmax k] [--nnbmax k] [--send] [--prec p] "
                 "[--rs0 x] [--rs ac|simple|mlp] [--mlp weights] "
                 "[--nnbmin x] [--log-train file] [--log-max n] "
-                "[--log-seed s] [--progress seconds]\n",
+                "[--log-seed s] [--progress seconds] [--irr device|host] "
+                "[--irr-min k]\n",
                 argv[0]);
         exit(1);
     }
@@ -1012,6 +1079,16 @@ void parse_args(int argc, char **argv, Opts &o, std::string &in) {
         else if (s == "--log-max") o.log_max = atoll(nxt());
         else if (s == "--log-seed") o.log_seed = (unsigned)atol(nxt());
         else if (s == "--progress") o.progress = atof(nxt());
+        else if (s == "--irr-min") o.irr_min = atoi(nxt());
+        else if (s == "--irr") {
+            const std::string r = nxt();
+            if (r == "device") o.irr_dev = 1;
+            else if (r == "host") o.irr_dev = 0;
+            else {
+                fprintf(stderr, "unknown --irr %s\n", r.c_str());
+                exit(1);
+            }
+        }
         else {
             fprintf(stderr, "unknown option %s\n", s.c_str());
             exit(1);
@@ -1060,6 +1137,11 @@ int main(int argc, char **argv) {
     int nbmax = n + 10, irank = 0;
     gpunb_open_(&nbmax, &irank);
     R.abi_check();
+    if (o.irr_dev) {
+        int nm = n, lm = o.lmax;
+        irr_simd_open_(&nm, &lm, &irank);
+        R.abi_check();
+    }
     if (o.rs_rule == 2) {
         if (o.mlp_path.empty()) {
             fprintf(stderr, "--rs mlp needs --mlp weights\n");
@@ -1105,25 +1187,27 @@ int main(int argc, char **argv) {
                    "\"dE_E\": %.6e, \"blocks\": %lld, \"irr_steps\": %lld, "
                    "\"reg_steps\": %lld, \"reg_calls\": %lld, \"reg_pairs\": %lld, "
                    "\"irr_pairs\": %lld, \"overflows\": %lld, \"nnb_mean\": %.2f, "
-                   "\"nnb_dev\": %.4f, \"rs_rule\": \"%s\", "
+                   "\"nnb_dev\": %.4f, \"rs_rule\": \"%s\", \"irr\": \"%s\", "
                    "\"learned_steps\": %lld, \"fallbacks\": %lld, "
                    "\"log_seen\": %lld, \"rscale\": %.4f, \"rcore\": %.4f, "
                    "\"t_reg\": %.3f, \"t_irr\": %.3f, \"t_pred\": %.3f, "
-                   "\"t_pot\": %.3f, \"t_mlp\": %.3f, \"wall\": %.3f}\n",
+                   "\"t_pot\": %.3f, \"t_mlp\": %.3f, \"t_jp\": %.3f, \"wall\": %.3f}\n",
                    R.t, e, ek, ep, (e - e0) / std::fabs(e0), R.blocks,
                    R.irr_steps, R.reg_steps, R.reg_calls, R.reg_pairs,
                    R.irr_pairs, R.overflows, (double)nnbs / n,
                    R.nnb_dev_n ? R.nnb_dev_sum / R.nnb_dev_n : 0.0,
                    o.rs_rule == 0 ? "ac" : o.rs_rule == 1 ? "simple" : "mlp",
+                   o.irr_dev ? "device" : "host",
                    R.learned_steps, R.fallbacks, R.log_seen, R.rscale,
                    std::sqrt(R.rc2), R.t_reg, R.t_irr, R.t_pred, R.t_pot,
-                   R.t_mlp, wall);
+                   R.t_mlp, R.t_jp, wall);
             fflush(stdout);
             next += o.dtadj;
             R.update_centre();
         }
     }
     R.write_log();
+    if (o.irr_dev) irr_simd_close_(&irank);
     gpunb_close_();
     return 0;
 }

@@ -19,6 +19,8 @@
  *   orc_nearest  src/Main/nbint.F:29-47 (IMINR: strict <, first in list order).
  *   orc_pot      src/Main/gpupot.gpu.cu:32-57, src/Main/pot.avx.cpp:87-137.
  *   orc_predict  src/Main/xbpredall.f:18-26.
+ *   orc_irr      src/Main/irr.avx.cpp:406-438 (irr_simd_firr) with the
+ *                predictor polynomial of xbpredall.f:18-26, in FP64.
  *
  * How each is pinned (tests/test_oracle_*.py): closed forms, Newton's third
  * law, long-double brute force on small inputs, cubic-trajectory exactness,
@@ -227,5 +229,68 @@ void orc_regf_ld(int nj, const double *m, const double *xj, const double *vj,
             jrk[3 * (size_t)i + k] = (double)jk[k];
         }
         pot[i] = (double)ph;
+    }
+}
+
+/*
+ * Irregular force of ni particles (0-based indices idx[]) at time t over
+ * their neighbour lists (irr.avx.cpp:406-438, restated in FP64).  Every
+ * particle is predicted from its own (x0, v0, f = a/2, fd = jerk/6, t0) with
+ * the xbpredall.f polynomial (orc_predict's association).  Lists: row i of
+ * `list` (stride lstride) holds nnb then nnb 0-based indices.
+ * acc/jrk per pair: rinv = 1/sqrt(r2), mr3 = m rinv^3,
+ * alpha = -3 (dx.dv) rinv^2, acc += mr3 dx, jrk += mr3 (dv + alpha dx),
+ * summed in list order; a pair with r2 == 0 is skipped.
+ * nnbid = 1 + the member with the smallest r2 (ties: the smaller index),
+ * 0 for an empty list (the reference's packed (r2, index) minimum).
+ */
+void orc_irr(int ni, const int *idx, double t, const double *t0,
+             const double *x0, const double *v0, const double *f,
+             const double *fd, const double *m, const int *list, int lstride,
+             double *acc, double *jrk, int *nnbid) {
+    for (int q = 0; q < ni; q++) {
+        const int i = idx[q];
+        double xi[3], vi[3];
+        orc_predict(1, t, t0 + i, x0 + 3 * (size_t)i, v0 + 3 * (size_t)i,
+                    f + 3 * (size_t)i, fd + 3 * (size_t)i, xi, vi);
+        const int *row = list + (size_t)i * lstride;
+        const int nnb = row[0];
+        double a[3] = {0.0, 0.0, 0.0}, jk[3] = {0.0, 0.0, 0.0};
+        double best = INFINITY;
+        int bestj = -1;
+        for (int k = 0; k < nnb; k++) {
+            const int j = row[1 + k];
+            double xj[3], vj[3];
+            orc_predict(1, t, t0 + j, x0 + 3 * (size_t)j, v0 + 3 * (size_t)j,
+                        f + 3 * (size_t)j, fd + 3 * (size_t)j, xj, vj);
+            const double dx = xj[0] - xi[0], dy = xj[1] - xi[1],
+                         dz = xj[2] - xi[2];
+            const double dvx = vj[0] - vi[0], dvy = vj[1] - vi[1],
+                         dvz = vj[2] - vi[2];
+            const double r2 = sq3(dx, dy, dz);
+            if (r2 < best || (r2 == best && j < bestj)) {
+                best = r2;
+                bestj = j;
+            }
+            if (r2 == 0.0) continue;
+            const double rinv = 1.0 / sqrt(r2);
+            const double rinv2 = rinv * rinv;
+            const double mr3 = m[j] * rinv2 * rinv;
+            double rv = dx * dvx;
+            rv = rv + dy * dvy;
+            rv = rv + dz * dvz;
+            const double alpha = -3.0 * rv * rinv2;
+            a[0] += mr3 * dx;
+            a[1] += mr3 * dy;
+            a[2] += mr3 * dz;
+            jk[0] += mr3 * (dvx + alpha * dx);
+            jk[1] += mr3 * (dvy + alpha * dy);
+            jk[2] += mr3 * (dvz + alpha * dz);
+        }
+        for (int c = 0; c < 3; c++) {
+            acc[3 * (size_t)q + c] = a[c];
+            jrk[3 * (size_t)q + c] = jk[c];
+        }
+        nnbid[q] = bestj + 1;
     }
 }

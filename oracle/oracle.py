@@ -7,6 +7,7 @@ Function map (reference tree paths):
   pot      src/Main/gpupot.gpu.cu:32-57; src/Main/pot.avx.cpp:87-137
   predict  src/Main/xbpredall.f:18-26
   nearest  src/Main/nbint.F:29-47
+  irr      src/Main/irr.avx.cpp:406-438 (with the xbpredall.f predictor)
 """
 from __future__ import annotations
 
@@ -53,6 +54,10 @@ def load():
         lib.orc_pair_is_nb.argtypes = [_dp, _dp, ctypes.c_double,
                                        ctypes.c_double, _dp, _dp,
                                        ctypes.c_double, ctypes.c_int, _dp]
+        lib.orc_irr.restype = None
+        lib.orc_irr.argtypes = [ctypes.c_int, _ip, ctypes.c_double, _dp, _dp,
+                                _dp, _dp, _dp, _dp, _ip, ctypes.c_int, _dp,
+                                _dp, _ip]
         lib.orc_regf_ld.restype = None
         lib.orc_regf_ld.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_int,
                                     _dp, _dp, ctypes.c_void_p, _dp, _dp, _dp]
@@ -109,6 +114,28 @@ def predict(t, t0, x0, v0, f, fd):
     load().orc_predict(n, float(t), _p(t0), *[_p(a) for a in arrs], _p(x),
                        _p(v))
     return x, v
+
+
+def irr(t, idx, t0, x0, v0, f, fd, m, lists):
+    """Irregular force at time t of the particles idx (0-based) over their
+    lists (lists[i] = sequence of 0-based neighbour indices of i).
+    Returns (acc (ni, 3), jrk (ni, 3), nnbid (ni,), 1-based, 0 if empty)."""
+    t0 = _d(t0)
+    n = t0.shape[0]
+    arrs = [_d(a, (n, 3)) for a in (x0, v0, f, fd)]
+    stride = 1 + max([len(l) for l in lists] + [0])
+    L = np.zeros((n, stride), dtype=np.int32)
+    for i, l in enumerate(lists):
+        L[i, 0] = len(l)
+        L[i, 1:1 + len(l)] = l
+    idx = np.ascontiguousarray(idx, dtype=np.int32)
+    ni = idx.shape[0]
+    acc = np.zeros((ni, 3))
+    jrk = np.zeros((ni, 3))
+    nnbid = np.zeros(ni, dtype=np.int32)
+    load().orc_irr(ni, _p(idx), float(t), _p(t0), *[_p(a) for a in arrs],
+                   _p(_d(m)), _p(L), stride, _p(acc), _p(jrk), _p(nnbid))
+    return acc, jrk, nnbid
 
 
 def nearest(xi, idx, xj):

@@ -1,6 +1,6 @@
 """ctypes front end of the reference's own CPU libraries, compiled from the
 reference sources by oracle/Makefile into oracle/_ref/ (reg.avx.cpp,
-reg.sse.cpp, pot.avx.cpp, pot.sse.cpp).  TEST INFRASTRUCTURE ONLY: used to
+reg.sse.cpp, pot.avx.cpp, pot.sse.cpp, irr.avx.cpp).  TEST INFRASTRUCTURE ONLY: used to
 pin the oracle (tests/) and as the timed CPU baseline (bench.py).
 
 Each library exports the same Fortran ABI as the engine (gpunb_open_ ...);
@@ -46,7 +46,16 @@ def load(name: str):
         if not os.path.exists(p):
             raise FileNotFoundError(p)
         lib = ctypes.CDLL(p, mode=os.RTLD_LOCAL | os.RTLD_NOW)
-        if name.startswith("reg"):
+        if name.startswith("irr"):
+            lib.irr_simd_open_.argtypes = [_ip, _ip, _ip]
+            lib.irr_simd_close_.argtypes = [_ip]
+            lib.irr_simd_set_jp_.argtypes = [_ip, _dp, _dp, _dp, _dp, _dp, _dp]
+            lib.irr_simd_set_list_.argtypes = [_ip, _ip]
+            lib.irr_simd_firr_vec_.argtypes = [_dp, _ip, _ip, _dp, _dp, _ip]
+            for f in ("irr_simd_open_", "irr_simd_close_", "irr_simd_set_jp_",
+                      "irr_simd_set_list_", "irr_simd_firr_vec_"):
+                getattr(lib, f).restype = None
+        elif name.startswith("reg"):
             lib.gpunb_open_.argtypes = [_ip, _ip]
             lib.gpunb_close_.argtypes = []
             lib.gpunb_send_.argtypes = [_ip, _dp, _dp, _dp]
@@ -113,6 +122,57 @@ class RegLib:
                              _i(lmax), _i(nnbmax), lst.ctypes.data_as(_ip),
                              _i(m_flag))
         return acc[:ni], jrk[:ni], pot[:ni], lst[:ni]
+
+
+class IrrLib:
+    """The reference's irregular-force library (irr.avx.cpp): predictor
+    state and lists per particle (1-based addresses, as its Fortran callers
+    pass them), FP32 pair arithmetic on double-single positions."""
+
+    def __init__(self, name: str = "irr_avx", nmax: int = 0, lmax: int = 601):
+        self.lib = load(name)
+        self.lmax = lmax
+        self.lib.irr_simd_open_(_i(nmax), _i(lmax), _i(0))
+
+    def close(self):
+        self.lib.irr_simd_close_(_i(0))
+
+    def set_jp(self, t0, x0, v0, f, fd, m):
+        """All particles 1..n: x0, v0, f = a/2, fd = jerk/6, mass, t0."""
+        t0 = _d(t0)
+        n = t0.shape[0]
+        x0, v0, f, fd = (_d(a, (n, 3)) for a in (x0, v0, f, fd))
+        m = _d(m)
+        for i in range(n):
+            self.lib.irr_simd_set_jp_(
+                _i(i + 1), x0[i].ctypes.data_as(_dp), v0[i].ctypes.data_as(_dp),
+                f[i].ctypes.data_as(_dp), fd[i].ctypes.data_as(_dp),
+                m[i:i + 1].ctypes.data_as(_dp), t0[i:i + 1].ctypes.data_as(_dp))
+
+    def set_list(self, i, members):
+        """List of particle i (0-based) = members (0-based indices); the row
+        is padded as a Fortran LIST(LMAX) row (the library reads whole
+        groups of 8)."""
+        row = np.zeros(max(self.lmax, len(members) + 9), dtype=np.int32)
+        row[0] = len(members)
+        row[1:1 + len(members)] = np.asarray(members) + 1
+        self.lib.irr_simd_set_list_(_i(i + 1), row.ctypes.data_as(_ip))
+
+    def firr(self, t, idx):
+        """(acc, jrk, nnbid) of the particles idx (0-based) at time t;
+        nnbid is 1-based."""
+        idx = np.ascontiguousarray(np.asarray(idx) + 1, dtype=np.int32)
+        ni = idx.shape[0]
+        acc = np.zeros((ni, 3))
+        jrk = np.zeros((ni, 3))
+        nn = np.zeros(ni, dtype=np.int32)
+        tt = ctypes.c_double(float(t))
+        self.lib.irr_simd_firr_vec_(ctypes.byref(tt), _i(ni),
+                                    idx.ctypes.data_as(_ip),
+                                    acc.ctypes.data_as(_dp),
+                                    jrk.ctypes.data_as(_dp),
+                                    nn.ctypes.data_as(_ip))
+        return acc, jrk, nn
 
 
 def pot(name: str, m, x, istart: int = 1, ni: int = None):

@@ -1,0 +1,175 @@
+"""Irregular force on the device (irr_simd_* drop-in) against the FP64
+oracle (oracle.irr) and the reference's own AVX library.
+
+Bar: nearest-neighbour indices identical to the oracle's (both compute r^2
+with the same FP64 operations); acceleration and jerk within 1e-12 of the
+per-particle sum of absolute pair terms (the FP64 summation-order bound is
+below 1e-13 for lists of at most 600); against the reference's FP32 library,
+the oracle test's FP32 bar.
+"""
+import numpy as np
+import pytest
+from scipy.spatial import cKDTree
+
+from nbody6ppgpu_amd import gpunb, ics
+from oracle import oracle, ref
+
+pytestmark = pytest.mark.gpu
+
+
+def state(n, seed):
+    rng = np.random.default_rng(seed)
+    c = ics.plummer(n, seed=seed, m=ics.kroupa(n, seed=seed + 1))
+    t0 = -rng.uniform(0, 1 / 256, n)
+    f = rng.normal(0, 0.5, (n, 3))
+    fd = rng.normal(0, 2.0, (n, 3))
+    return c.m, c.x, c.v, f, fd, t0
+
+
+def knn_lists(x, k):
+    _, idx = cKDTree(x).query(x, k=k + 1)
+    return [list(np.sort(r[1:])) for r in idx]
+
+
+def scales(t, idx, t0, x, v, f, fd, m, lists):
+    xp, vp = oracle.predict(t, t0, x, v, f, fd)
+    sa, sj = [], []
+    for i in idx:
+        l = np.asarray(lists[i], dtype=int)
+        if l.size == 0:
+            sa.append(0.0)
+            sj.append(0.0)
+            continue
+        d = xp[l] - xp[i]
+        u = vp[l] - vp[i]
+        r2 = np.sum(d * d, axis=1)
+        ok = r2 > 0
+        r = np.sqrt(r2[ok])
+        sa.append(np.sum(m[l][ok] / r2[ok]))
+        sj.append(np.sum(m[l][ok] * (np.linalg.norm(u[ok], axis=1) / r ** 3
+                                      + 3 * np.abs(np.sum(d[ok] * u[ok], axis=1)) / r ** 4)))
+    return np.array(sa), np.array(sj)
+
+
+def upload(n, m, x, v, f, fd, t0, lists, lmax):
+    gpunb.irr_open(n, lmax)
+    addr = np.arange(1, n + 1)
+    gpunb.irr_set_jp(addr, x, v, f, fd, m, t0)
+    gpunb.irr_set_lists(addr, [np.asarray(l, dtype=np.int32) + 1 for l in lists])
+
+
+def check(got, want, sc):
+    (ga, gj, gn), (oa, oj, on), (sa, sj) = got, want, sc
+    assert np.array_equal(gn, on)
+    assert np.all(np.linalg.norm(ga - oa, axis=1) <= 1e-12 * sa + 1e-300)
+    assert np.all(np.linalg.norm(gj - oj, axis=1) <= 1e-12 * sj + 1e-300)
+
+
+def test_matches_oracle_with_edge_lists():
+    n = 4000
+    m, x, v, f, fd, t0 = state(n, 11)
+    lists = knn_lists(x, 48)
+    lists[5] = []                                    # empty list
+    lists[6] = list(range(7, 7 + 300))               # long list
+    x[9] = x[8]                                      # a coincident pair
+    v[9] = v[8]; f[9] = f[8]; fd[9] = fd[8]; t0[9] = t0[8]
+    lists[8] = sorted(set(lists[8]) | {9})
+    m[20] = 0.0                                      # a massless member
+    upload(n, m, x, v, f, fd, t0, lists, 601)
+    try:
+        idx = np.concatenate([[5, 6, 8], np.arange(100, n, 7)])
+        t = 0.003
+        got = gpunb.irr_firr(t, idx + 1)
+        want = oracle.irr(t, idx, t0, x, v, f, fd, m, lists)
+        check(got, want, scales(t, idx, t0, x, v, f, fd, m, lists))
+        assert got[2][0] == 0 and not got[0][0].any()
+        # ni = 0 is a no-op
+        a, j, nn = gpunb.irr_firr(t, np.zeros(0, np.int32))
+        assert a.shape == (0, 3)
+    finally:
+        gpunb.irr_close()
+
+
+def test_fortran_calls_and_restaging():
+    """Per-particle irr_simd_set_jp_/set_list_ calls (the Fortran form)
+    give the batched results; a value set twice before firr_vec: last wins."""
+    import ctypes
+    n = 700
+    m, x, v, f, fd, t0 = state(n, 12)
+    lists = knn_lists(x, 30)
+    lib = gpunb.load()
+    gpunb.irr_open(n, 64)
+    try:
+        ip = lambda k: ctypes.byref(ctypes.c_int(int(k)))
+        dp = lambda a: np.ascontiguousarray(a, dtype=np.float64).ctypes.data_as(
+            ctypes.POINTER(ctypes.c_double))
+        for i in range(n):
+            # a wrong state first, then the right one
+            for xx in (x[i] + 1.0, x[i]):
+                lib.irr_simd_set_jp_(ip(i + 1), dp(xx), dp(v[i]), dp(f[i]),
+                                     dp(fd[i]), dp([m[i]]), dp([t0[i]]))
+            for l in (lists[(i + 1) % n], lists[i]):
+                row = np.zeros(64, np.int32)
+                row[0] = len(l)
+                row[1:1 + len(l)] = np.asarray(l) + 1
+                lib.irr_simd_set_list_(ip(i + 1), row.ctypes.data_as(
+                    ctypes.POINTER(ctypes.c_int)))
+        gpunb._check()
+        idx = np.arange(n)
+        got = gpunb.irr_firr(0.001, idx + 1)
+        want = oracle.irr(0.001, idx, t0, x, v, f, fd, m, lists)
+        check(got, want, scales(0.001, idx, t0, x, v, f, fd, m, lists))
+        # Update a subset of the state and lists, recompute.
+        sub = np.arange(0, n, 5)
+        x2 = x.copy()
+        x2[sub] += 1e-3
+        gpunb.irr_set_jp(sub + 1, x2[sub], v[sub], f[sub], fd[sub], m[sub], t0[sub])
+        lists2 = list(lists)
+        for i in sub[:20]:
+            lists2[i] = lists[i][:10]
+        gpunb.irr_set_lists(sub[:20] + 1, [np.asarray(lists2[i]) + 1 for i in sub[:20]])
+        got = gpunb.irr_firr(0.002, idx + 1)
+        want = oracle.irr(0.002, idx, t0, x2, v, f, fd, m, lists2)
+        check(got, want, scales(0.002, idx, t0, x2, v, f, fd, m, lists2))
+    finally:
+        gpunb.irr_close()
+
+
+def test_argument_errors():
+    gpunb.irr_open(100, 16)
+    try:
+        with pytest.raises(gpunb.GpunbError):
+            gpunb.irr_set_lists([1], [np.arange(2, 20)])      # 18 > lmax - 1
+        with pytest.raises(gpunb.GpunbError):
+            gpunb.irr_set_lists([1], [np.array([101])])       # member > nmax
+        with pytest.raises(gpunb.GpunbError):
+            gpunb.irr_firr(0.0, [0])                          # address 0
+    finally:
+        gpunb.irr_close()
+
+
+@pytest.mark.skipif(not ref.available("irr_avx"),
+                    reason="reference AVX irregular library not built")
+def test_matches_reference_avx_library():
+    n = 3000
+    m, x, v, f, fd, t0 = state(n, 13)
+    lists = knn_lists(x, 48)
+    R = ref.IrrLib("irr_avx", n, 601)
+    try:
+        R.set_jp(t0, x, v, f, fd, m)
+        for i, l in enumerate(lists):
+            R.set_list(i, l)
+        idx = np.arange(n)
+        ra, rj, rn = R.firr(0.002, idx)
+    finally:
+        R.close()
+    upload(n, m, x, v, f, fd, t0, lists, 601)
+    try:
+        ga, gj, gn = gpunb.irr_firr(0.002, idx + 1)
+    finally:
+        gpunb.irr_close()
+    rel_a = np.linalg.norm(ra - ga, axis=1) / np.linalg.norm(ga, axis=1)
+    rel_j = np.linalg.norm(rj - gj, axis=1) / np.linalg.norm(gj, axis=1)
+    assert np.median(rel_a) < 1e-6 and np.percentile(rel_a, 99) < 1e-4
+    assert np.median(rel_j) < 1e-5
+    assert np.mean(rn == gn) > 0.999
