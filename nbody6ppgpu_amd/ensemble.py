@@ -116,8 +116,11 @@ def autotune(b, nnbmax, reps=3, log=None):
     """Returns (group, rounds, bucket_tiles, best_seconds)."""
     best = (None, None, [], 1e30)
     gpunb.gpunb_ens_set_bucket_tiles([])
+    # Small clusters need many more slices per resident wave than one large
+    # cluster to even out the launch tail (measured on 64 clusters of
+    # 1024..65536: 6.8e11 pairs/s at 4 rounds, 1.05e12 at 64).
     for group in (2, 4):
-        for rounds in (1.0, 2.0, 4.0):
+        for rounds in (4.0, 16.0, 64.0, 128.0):
             gpunb.gpunb_set_tuning(group, rounds)
             t = time_step(b, nnbmax, "graph", reps)
             if log:
