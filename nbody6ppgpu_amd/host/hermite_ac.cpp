@@ -276,6 +276,7 @@ struct Run {
         const int k = (int)idx.size();
         std::vector<int> addr(k);
         std::vector<double> X(3 * k), V(3 * k), F(3 * k), FD(3 * k), M(k), T(k);
+#pragma omp parallel for schedule(static) if (k >= 4096)
         for (int q = 0; q < k; ++q) {
             const Body &b = B[idx[q]];
             addr[q] = idx[q] + 1;
@@ -758,6 +759,7 @@ struct Run {
         const int na = (int)A.size();
         // Predicted position of every active body at tn.
         std::vector<double> XP(3 * (size_t)na), VP(3 * (size_t)na);
+#pragma omp parallel for schedule(static) if (na >= 4096)
         for (int k = 0; k < na; ++k)
             predict_cubic(B[A[k]], tn, &XP[3 * (size_t)k], &VP[3 * (size_t)k]);
         // Irregular force with the current lists.
