@@ -77,12 +77,35 @@ __device__ inline void set_pad(JRec &r) {
 }
 
 // Fold the per-thread speed and mass extrema of one wave into JStats.
+// Workgroup-wide max |v| / min positive m folded into *jst.  Called by every
+// thread of a 256-thread workgroup (it synchronises).  The bounds only grow
+// (vmax) or shrink (mmin), so an update that the stored value already covers
+// is skipped: at most one atomic per workgroup and quantity, and after the
+// first workgroups almost none (same-address atomics serialise in L2).
 __device__ inline void stats_fold(JStats *jst, double vmag, double mpos) {
+    __shared__ double s_v[4], s_m[4];
     vmag = wave_max(vmag);
     mpos = wave_min(mpos);
+    const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
-        if (vmag > 0.0) atomicMax(&jst->vmax_bits, dbits(vmag));
-        if (mpos < INFINITY) atomicMin(&jst->mmin_bits, dbits(mpos));
+        s_v[w] = vmag;
+        s_m[w] = mpos;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = (int)(blockDim.x >> 6);
+        for (int k = 1; k < nw; ++k) {
+            vmag = fmax(vmag, s_v[k]);
+            mpos = fmin(mpos, s_m[k]);
+        }
+        if (vmag > 0.0 &&
+            dbits(vmag) > __hip_atomic_load(&jst->vmax_bits, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(&jst->vmax_bits, dbits(vmag));
+        if (mpos < INFINITY &&
+            dbits(mpos) < __hip_atomic_load(&jst->mmin_bits, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT))
+            atomicMin(&jst->mmin_bits, dbits(mpos));
     }
 }
 

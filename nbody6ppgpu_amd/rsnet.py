@@ -181,20 +181,22 @@ def summarise(recs):
 
 def experiment(n=16384, t_log=0.25, t_run=0.25, seed_train=31, seed_eval=37,
                log_max=2_000_000, epochs=20, nnbopt=48, etai=0.05, etar=0.1,
-               timeout=1200, workdir=None, log=None):
+               timeout=1200, workdir=None, log=None, nnbmax=0):
     """Train on one segregated Plummer realisation, then run another with
     three radius rules: ``ac`` (the reference's Ahmad-Cohen rule, which lowers
     the membership outside the half-mass radius), ``simple`` (volume scaling
     towards NNBOPT, the same target as the network's, without learning) and
     ``learned``.  The pairs are logged from a ``simple`` run, whose lists
     straddle NNBOPT, so about half of the targets are exact rather than
-    censored.  Files go to ``workdir`` (a temporary directory, removed
-    afterwards, when None).  Returns a dict (fit statistics, the three
-    runs)."""
+    censored.  ``nnbmax`` > 0 caps the lists of the three compared runs
+    (the logging run keeps the default), to measure overflow rates.  Files go
+    to ``workdir`` (a temporary directory, removed afterwards, when None).
+    Returns a dict (fit statistics, the three runs)."""
     if workdir is None:
         with tempfile.TemporaryDirectory(prefix="rsnet_") as td:
             return experiment(n, t_log, t_run, seed_train, seed_eval, log_max,
-                              epochs, nnbopt, etai, etar, timeout, td, log)
+                              epochs, nnbopt, etai, etar, timeout, td, log,
+                              nnbmax)
     from . import ics, integrate
     # Energy checks every dtadj (a power of two no longer than either run).
     dtadj = 0.125
@@ -220,6 +222,8 @@ def experiment(n=16384, t_log=0.25, t_run=0.25, seed_train=31, seed_eval=37,
     runs = {}
     for name, extra in (("ac", ("--rs", "ac")), ("simple", ("--rs", "simple")),
                         ("learned", ("--rs", "mlp", "--mlp", wp))):
+        if nnbmax > 0:
+            extra = extra + ("--nnbmax", str(nnbmax))
         recs = integrate.run(ce.m, ce.x, ce.v, t_end=t_run, etai=etai,
                              etar=etar, dtadj=dtadj, nnbopt=nnbopt,
                              timeout=timeout, extra=extra, progress=log)
@@ -230,7 +234,8 @@ def experiment(n=16384, t_log=0.25, t_run=0.25, seed_train=31, seed_eval=37,
     fit_stats["pairs_logged"] = int(F.shape[0])
     fit_stats["censored_frac"] = float(cens.mean())
     fit_stats["log_run_seconds"] = t_logrun
-    return {"n": n, "fit": fit_stats, "runs": runs, "workdir": td}
+    return {"n": n, "nnbmax": nnbmax or None, "fit": fit_stats, "runs": runs,
+            "workdir": td}
 
 
 def kernel_bench(n_train=2_000_000, n_pred=1_048_576, epochs=3, batch=1024,
@@ -269,6 +274,8 @@ def main(argv=None):
                     help="N-body time units of the logging and compared runs")
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--log-max", type=int, default=2_000_000)
+    ap.add_argument("--nnbmax", type=int, default=0,
+                    help="list cap of the compared runs (0: the driver default)")
     ap.add_argument("--kernel-bench", action="store_true",
                     help="time the network kernels alone on synthetic pairs "
                          "(2M training pairs, 3 epochs; 1M predictions)")
@@ -278,7 +285,7 @@ def main(argv=None):
         return
     t0 = time.perf_counter()
     out = experiment(n=a.n, t_log=a.t, t_run=a.t, epochs=a.epochs,
-                     log_max=a.log_max,
+                     log_max=a.log_max, nnbmax=a.nnbmax,
                      log=lambda s: print(f"[{time.perf_counter() - t0:7.1f}s] {s}",
                                          file=sys.stderr, flush=True))
     out.pop("workdir")

@@ -38,3 +38,25 @@ def test_king_kroupa_16k_benchmark_steps():
                         timeout=600)
     assert rec[-1]["t"] == pytest.approx(1.0)
     assert summary("King 16k Kroupa eta 0.05/0.1", rec) < 2e-4
+
+
+def test_device_irregular_force_tracks_host():
+    """Every block's irregular force on the device (irr_simd_firr_vec_, FP64)
+    against the host's FP64 sum: the two differ only in summation order and
+    prediction association, so the trajectories agree closely over a short
+    run (chaos amplifies rounding differences only slowly at t = 1/8)."""
+    c = ics.king(8192, 6.0, seed=404, m=ics.kroupa(8192, seed=405))
+    runs = {}
+    for mode, extra in (("device", ("--irr", "device", "--irr-min", "0")),
+                        ("host", ("--irr", "host"))):
+        rec = integrate.run(c.m, c.x, c.v, t_end=0.125, etai=0.05, etar=0.1,
+                            dtadj=0.0625, timeout=600, extra=extra)
+        assert rec[-1]["irr"] == mode
+        runs[mode] = rec
+    d, h = runs["device"][-1], runs["host"][-1]
+    # Step choices are quantised to powers of two; a rounding-level
+    # difference may flip one near a boundary, so allow a few.
+    assert abs(d["irr_steps"] - h["irr_steps"]) <= 1e-3 * h["irr_steps"]
+    assert abs(d["reg_steps"] - h["reg_steps"]) <= 1e-3 * h["reg_steps"]
+    assert abs(d["E"] - h["E"]) <= 1e-9 * abs(h["E"])
+    assert abs(d["dE_E"]) < 1e-5 and abs(h["dE_E"]) < 1e-5
