@@ -21,6 +21,7 @@ timed on the host cores.
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -191,15 +192,44 @@ def main():
         f"pair kernel {k_launches} launches, mean {avg_launch_s * 1e3:.3f} ms")
 
     # Drop-in ABI path with host arrays (PCIe-inclusive), i in chunks of
-    # 1024 as the Fortran host calls it; reported next to value, never as it.
+    # 1024 as the Fortran host calls it (util_gpu.F:33-37), into output
+    # arrays allocated once as the host's LISTGP/FREG arrays are; reported
+    # next to value, never as it.
     gpunb.gpunb_send(c.m, c.x, c.v)
-    nchunk = 32
+    nchunk, ci = 32, 1024
+    lmax_abi = nnbmax + 1
+    X = np.ascontiguousarray(c.x)
+    V = np.ascontiguousarray(c.v)
+    H2 = np.ascontiguousarray(h2)
+    DT = np.ascontiguousarray(stepr)
+    acc_o = np.zeros((ci, 3))
+    jrk_o = np.zeros((ci, 3))
+    pot_o = np.zeros(ci)
+    lst_o = np.zeros((ci, lmax_abi), dtype=np.int32)
+    lib = gpunb.load()
+    dptr = ctypes.POINTER(ctypes.c_double)
+
+    def rows(a, first):
+        """Pointer to row `first` of a C-contiguous FP64 array."""
+        return ctypes.cast(a.ctypes.data + first * a.strides[0], dptr)
+
+    ni_c, lmax_c, nnb_c, mf_c = (ctypes.c_int(v) for v in (ci, lmax_abi, nnbmax, 0))
+    outs = [rows(acc_o, 0), rows(jrk_o, 0), rows(pot_o, 0)]
+    lst_p = lst_o.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+    def call(q):
+        o = q * ci
+        lib.gpunb_regf_(ctypes.byref(ni_c), rows(H2, o), rows(DT, o),
+                        rows(X, o), rows(V, o), *outs, ctypes.byref(lmax_c),
+                        ctypes.byref(nnb_c), lst_p, ctypes.byref(mf_c))
+
+    call(0)
+    gpunb._check()
     t2 = time.perf_counter()
     for q in range(nchunk):
-        sl = slice(q * 1024, (q + 1) * 1024)
-        gpunb.gpunb_regf(h2[sl], stepr[sl], c.x[sl], c.v[sl], nnbmax + 1,
-                         nnbmax, 0)
-    abi_rate = nchunk * 1024 * n / (time.perf_counter() - t2)
+        call(q)
+    abi_rate = nchunk * ci * n / (time.perf_counter() - t2)
+    gpunb._check()
     gpunb.gpunb_close()
 
     cpu = None

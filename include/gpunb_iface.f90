@@ -75,5 +75,37 @@ module gpunb_iface
       integer(c_int), intent(in) :: n
       real(c_double), intent(out) :: x(3,*), v(3,*)
     end subroutine
+    ! Irregular force, src/Main/irr.avx.cpp:565-603 (callers intgrt.F:175,
+    ! 202-204, 545, 971, 1194, 1270, 1502).  Addresses and members 1-based.
+    subroutine irr_simd_open(nmax, lmax, rank) bind(C, name="irr_simd_open_")
+      import :: c_int
+      integer(c_int), intent(in) :: nmax, lmax, rank
+    end subroutine
+    subroutine irr_simd_close(rank) bind(C, name="irr_simd_close_")
+      import :: c_int
+      integer(c_int), intent(in) :: rank
+    end subroutine
+    subroutine irr_simd_profile(rank) bind(C, name="irr_simd_profile_")
+      import :: c_int
+      integer(c_int), intent(in) :: rank
+    end subroutine
+    subroutine irr_simd_set_jp(addr, pos, vel, acc2, jrk6, mass, time) &
+        bind(C, name="irr_simd_set_jp_")
+      import :: c_int, c_double
+      integer(c_int), intent(in) :: addr
+      real(c_double), intent(in) :: pos(3), vel(3), acc2(3), jrk6(3), mass, time
+    end subroutine
+    subroutine irr_simd_set_list(addr, nblist) bind(C, name="irr_simd_set_list_")
+      import :: c_int
+      integer(c_int), intent(in) :: addr, nblist(*)
+    end subroutine
+    subroutine irr_simd_firr_vec(ti, ni, addr, acc, jrk, nnbid) &
+        bind(C, name="irr_simd_firr_vec_")
+      import :: c_int, c_double
+      real(c_double), intent(in) :: ti
+      integer(c_int), intent(in) :: ni, addr(*)
+      real(c_double), intent(out) :: acc(3,*), jrk(3,*)
+      integer(c_int), intent(out) :: nnbid(*)
+    end subroutine
   end interface
 end module gpunb_iface

@@ -18,6 +18,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <stdexcept>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -160,12 +161,16 @@ State S;
 bool g_abort = true;
 bool g_abort_set = false;
 std::string g_last_error;
+std::mutex g_error_mu;   // entry points may fail on several threads at once
 
 bool quiet() { return env_flag("GPUNB_QUIET", false); }
 
 void report(const char *fn, const char *msg) {
-    g_last_error = std::string(fn) + ": " + msg;
-    fprintf(stderr, "gpunb: %s\n", g_last_error.c_str());
+    {
+        std::lock_guard<std::mutex> g(g_error_mu);
+        g_last_error = std::string(fn) + ": " + msg;
+    }
+    fprintf(stderr, "gpunb: %s: %s\n", fn, msg);
     fflush(stderr);
     bool ab = g_abort_set ? g_abort : env_flag("GPUNB_ABORT", true);
     if (ab) abort();
@@ -1085,6 +1090,7 @@ GPUNB_EXPORT void gpunb_set_abort_(int *on) {
 }
 
 GPUNB_EXPORT int gpunb_last_error(char *buf, int len) {
+    std::lock_guard<std::mutex> g(g_error_mu);
     const int n = (int)g_last_error.size();
     if (buf && len > 0) {
         const int k = std::min(n, len - 1);

@@ -27,6 +27,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <ctime>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -222,6 +223,103 @@ struct State {
 };
 State I;
 
+// Staging.  The reference's host calls set_jp / set_list for every particle
+// inside an OpenMP parallel loop (intgrt.F:200-205), so they must be safe to
+// call concurrently: each thread appends to its own buffer, and the next
+// firr_vec (called from one thread, with no staging running) merges all
+// buffers, keeping the last value staged for each particle.
+struct Stage {
+    std::vector<IRec> rec;
+    std::vector<int> rec_addr;   // 0-based slot per record
+    std::vector<int> hdr;        // (slot, offset in flat, count) per row
+    std::vector<int> flat;       // members, 0-based
+    void clear() {
+        rec.clear();
+        rec_addr.clear();
+        hdr.clear();
+        flat.clear();
+    }
+};
+std::mutex g_stage_mu;
+std::vector<Stage *> g_stages;   // kept for the life of the process
+thread_local Stage *t_stage = nullptr;
+
+Stage &my_stage() {
+    if (!t_stage) {
+        Stage *st = new Stage;
+        std::lock_guard<std::mutex> g(g_stage_mu);
+        g_stages.push_back(st);
+        t_stage = st;
+    }
+    return *t_stage;
+}
+
+void stage_jp(int a, const double *x, const double *v, const double *f,
+              const double *fd, double m, double t0) {
+    IREQ(a >= 1 && a <= I.nmax, "address out of range [1, nmax]");
+    IRec r{};
+    for (int k = 0; k < 3; ++k) {
+        r.x[k] = x[k];
+        r.v[k] = v[k];
+        r.f[k] = f[k];
+        r.fd[k] = fd[k];
+    }
+    r.m = m;
+    r.t0 = t0;
+    Stage &st = my_stage();
+    st.rec.push_back(r);
+    st.rec_addr.push_back(a - 1);
+}
+
+// nb: 1-based member indices.
+void stage_list(int a, int cnt, const int *nb) {
+    IREQ(a >= 1 && a <= I.nmax, "address out of range [1, nmax]");
+    IREQ(cnt >= 0 && cnt <= I.lstride - 1, "list longer than lmax - 1");
+    for (int k = 0; k < cnt; ++k)
+        IREQ(nb[k] >= 1 && nb[k] <= I.nmax, "list member out of range [1, nmax]");
+    Stage &st = my_stage();
+    st.hdr.push_back(a - 1);
+    st.hdr.push_back((int)st.flat.size());
+    st.hdr.push_back(cnt);
+    for (int k = 0; k < cnt; ++k) st.flat.push_back(nb[k] - 1);
+}
+
+// All threads' staged data into I.jp_rec / jp_addr / ls_hdr / ls_flat, the
+// last value per particle winning (a superseded row keeps slot -1).
+void merge_stages() {
+    std::lock_guard<std::mutex> g(g_stage_mu);
+    for (Stage *st : g_stages) {
+        for (size_t k = 0; k < st->rec.size(); ++k) {
+            const int s = st->rec_addr[k];
+            if (I.jp_slot[s] >= 0) {
+                I.jp_rec[I.jp_slot[s]] = st->rec[k];
+            } else {
+                I.jp_slot[s] = (int)I.jp_rec.size();
+                I.jp_rec.push_back(st->rec[k]);
+                I.jp_addr.push_back(s);
+            }
+        }
+        for (size_t r = 0; r < st->hdr.size(); r += 3) {
+            const int s = st->hdr[r], off = st->hdr[r + 1], cnt = st->hdr[r + 2];
+            if (I.ls_slot[s] >= 0) I.ls_hdr[3 * I.ls_slot[s]] = -1;
+            I.ls_slot[s] = (int)(I.ls_hdr.size() / 3);
+            I.ls_hdr.push_back(s);
+            I.ls_hdr.push_back((int)I.ls_flat.size());
+            I.ls_hdr.push_back(cnt);
+            I.ls_flat.insert(I.ls_flat.end(), st->flat.begin() + off,
+                             st->flat.begin() + off + cnt);
+            I.nnb[s] = cnt;
+        }
+        st->clear();
+    }
+}
+
+void clear_stages() {
+    std::lock_guard<std::mutex> g(g_stage_mu);
+    for (Stage *st : g_stages) st->clear();
+}
+
+
 double wtime() {
     struct timespec ts;
     clock_gettime(CLOCK_MONOTONIC, &ts);
@@ -256,6 +354,7 @@ void do_open(int nmax, int lmax, int rank) {
     I.jp_addr.clear();
     I.ls_hdr.clear();
     I.ls_flat.clear();
+    clear_stages();
     I.ncall = I.nsteps = I.ninter = 0;
     I.t_force = 0.0;
     I.open = true;
@@ -286,51 +385,12 @@ void do_close(int rank) {
     if (!quiet()) fprintf(stderr, "Closing irr_simd lib. MI355X ver. - rank: %d\n", rank);
 }
 
-void stage_jp(int a, const double *x, const double *v, const double *f,
-              const double *fd, double m, double t0) {
-    IREQ(a >= 1 && a <= I.nmax, "address out of range [1, nmax]");
-    IRec r{};
-    for (int k = 0; k < 3; ++k) {
-        r.x[k] = x[k];
-        r.v[k] = v[k];
-        r.f[k] = f[k];
-        r.fd[k] = fd[k];
-    }
-    r.m = m;
-    r.t0 = t0;
-    const int s = a - 1;
-    if (I.jp_slot[s] >= 0) {
-        I.jp_rec[I.jp_slot[s]] = r;
-    } else {
-        I.jp_slot[s] = (int)I.jp_rec.size();
-        I.jp_rec.push_back(r);
-        I.jp_addr.push_back(s);
-    }
-}
-
-// nb: 1-based member indices.
-void stage_list(int a, int cnt, const int *nb) {
-    IREQ(a >= 1 && a <= I.nmax, "address out of range [1, nmax]");
-    IREQ(cnt >= 0 && cnt <= I.lstride - 1, "list longer than lmax - 1");
-    const int s = a - 1;
-    if (I.ls_slot[s] >= 0) I.ls_hdr[3 * I.ls_slot[s]] = -1;   // superseded
-    I.ls_slot[s] = (int)(I.ls_hdr.size() / 3);
-    I.ls_hdr.push_back(s);
-    I.ls_hdr.push_back((int)I.ls_flat.size());
-    I.ls_hdr.push_back(cnt);
-    for (int k = 0; k < cnt; ++k) {
-        const int j = nb[k];
-        IREQ(j >= 1 && j <= I.nmax, "list member out of range [1, nmax]");
-        I.ls_flat.push_back(j - 1);
-    }
-    I.nnb[s] = cnt;
-}
-
 // Upload the staged records and list rows, scatter them, then (ni > 0) the
 // force of addr[] at t; one host synchronisation at the end.  Layout of the
 // pinned int buffer: [jp slots (nj)] [row headers (3 nr)] [members] [i (ni)].
 void flush_and_force(double t, int ni, const int *addr, double *acc,
                      double *jrk, int *nnbid) {
+    merge_stages();
     const size_t nj = I.jp_rec.size();
     std::vector<int> hdr;
     hdr.reserve(I.ls_hdr.size());

@@ -1,9 +1,11 @@
 """The reference's Fortran host binds to this library unchanged.
 
-CPU: the ISO_C_BINDING module and the implicit-interface call both compile
+CPU: the ISO_C_BINDING module and the implicit-interface calls both compile
 with amdflang and resolve to exactly the symbols the library exports.
 GPU: the Fortran driver runs gpunb_open_/send_/regf_/close_ and gpupot_ in
-both binding styles; results are bit-identical to the same calls made from
+both binding styles, and the irregular-force library as intgrt.F drives it
+(IRR_SIMD_SET_JP / SET_LIST for every particle inside an OpenMP parallel
+loop, then FIRR_VEC); results are bit-identical to the same calls made from
 Python through the C-ABI and meet the oracle bar.
 """
 import os
@@ -28,9 +30,12 @@ def test_fortran_driver_links_to_library_symbols():
     if not os.path.exists(DRV):
         subprocess.check_call(["make", "-s", "-C", os.path.join(HERE, "fortran")])
     und = subprocess.run(["nm", "-u", DRV], capture_output=True, text=True).stdout
-    needed = {l.split()[-1] for l in und.splitlines() if "gpu" in l}
+    needed = {l.split()[-1] for l in und.splitlines()
+              if "gpu" in l or "irr_simd" in l}
     assert needed == {"gpunb_open_", "gpunb_send_", "gpunb_regf_",
-                      "gpunb_close_", "gpupot_"}
+                      "gpunb_close_", "gpupot_", "irr_simd_open_",
+                      "irr_simd_set_jp_", "irr_simd_set_list_",
+                      "irr_simd_firr_vec_", "irr_simd_close_"}
     exported = subprocess.run(["nm", "-D", "--defined-only",
                                gpunb.library_path()], capture_output=True,
                               text=True).stdout
@@ -49,10 +54,32 @@ def test_fortran_driver_matches_c_abi(tmp_path):
     n, lmax, nnbmax, mflag = c.n, 301, 300, 0
     fin = tmp_path / "in.bin"
     fout = tmp_path / "out.bin"
+    # Irregular-force block: predictor state, lists (1-based, LIST(LMAX2,N)
+    # rows: count then members) and the active particles.
+    rng = np.random.default_rng(303)
+    fa = rng.normal(0, 0.5, (n, 3))
+    fdot = rng.normal(0, 2.0, (n, 3))
+    t0 = -rng.uniform(0, 1 / 256, n)
+    from scipy.spatial import cKDTree
+    _, idx = cKDTree(c.x).query(c.x, k=33)
+    lists = [np.sort(r[1:]) for r in idx]
+    lists[7] = np.zeros(0, dtype=int)
+    lmax2, tirr = 64, 0.002
+    L2 = np.zeros((n, lmax2), dtype=np.int32)
+    for i, l in enumerate(lists):
+        L2[i, 0] = len(l)
+        L2[i, 1:1 + len(l)] = l + 1
+    act = np.arange(1, n + 1, 3, dtype=np.int32)
     with open(fin, "wb") as f:
         np.array([n, lmax, nnbmax, mflag], dtype=np.int32).tofile(f)
         for a in (c.m, c.x, c.v, h2, dtr):
             np.ascontiguousarray(a, dtype=np.float64).tofile(f)
+        np.array([lmax2, act.size], dtype=np.int32).tofile(f)
+        np.array([tirr], dtype=np.float64).tofile(f)
+        for a in (fa, fdot, t0):
+            np.ascontiguousarray(a, dtype=np.float64).tofile(f)
+        L2.tofile(f)
+        act.tofile(f)
     env = dict(os.environ, GPUNB_QUIET="1")
     subprocess.run([DRV, str(fin), str(fout)], check=True, env=env, timeout=120)
     raw = open(fout, "rb").read()
@@ -72,6 +99,11 @@ def test_fortran_driver_matches_c_abi(tmp_path):
         lst = take(np.int32, lmax * n).reshape(n, lmax)
         outs.append((acc, jrk, pot, lst))
     phi = take(np.float64, n)
+    ni2 = act.size
+    firr = take(np.float64, 3 * ni2).reshape(ni2, 3)
+    fdirr = take(np.float64, 3 * ni2).reshape(ni2, 3)
+    iminr = take(np.int32, ni2)
+    assert off == len(raw)
     gpunb.gpunb_open(n + 10)
     try:
         gpunb.gpunb_send(c.m, c.x, c.v)
@@ -91,4 +123,20 @@ def test_fortran_driver_matches_c_abi(tmp_path):
                   inputs=(c.m, c.x, c.v, c.x, c.v))
     op = oracle.pot(1, n, c.m, c.x)
     assert np.max(np.abs(phi - op) / op) < 1e-6
+    # Irregular force: the Fortran OpenMP staging gives the batched C-ABI's
+    # bits, and both meet the oracle (identical nearest neighbours).
+    gpunb.irr_open(n, lmax2)
+    try:
+        addr = np.arange(1, n + 1)
+        gpunb.irr_set_jp(addr, c.x, c.v, fa, fdot, c.m, t0)
+        gpunb.irr_set_lists(addr, [l + 1 for l in lists])
+        pa, pj, pn = gpunb.irr_firr(tirr, act)
+    finally:
+        gpunb.irr_close()
+    assert np.array_equal(firr, pa) and np.array_equal(fdirr, pj)
+    assert np.array_equal(iminr, pn)
+    oa, oj, on = oracle.irr(tirr, act - 1, t0, c.x, c.v, fa, fdot, c.m, lists)
+    assert np.array_equal(iminr, on)
+    scale = np.linalg.norm(oa, axis=1) + 1e-300
+    assert np.all(np.linalg.norm(firr - oa, axis=1) <= 1e-10 * scale + 1e-12 * np.max(scale))
     del nn

@@ -173,3 +173,43 @@ def test_matches_reference_avx_library():
     assert np.median(rel_a) < 1e-6 and np.percentile(rel_a, 99) < 1e-4
     assert np.median(rel_j) < 1e-5
     assert np.mean(rn == gn) > 0.999
+
+
+def test_concurrent_staging_from_threads():
+    """set_jp / set_list called from several threads at once, as the
+    reference's host does inside an OpenMP loop (intgrt.F:200-205)."""
+    import ctypes
+    from concurrent.futures import ThreadPoolExecutor
+    n = 3000
+    m, x, v, f, fd, t0 = state(n, 14)
+    lists = knn_lists(x, 40)
+    lib = gpunb.load()
+    dp = ctypes.POINTER(ctypes.c_double)
+    rows = {k: np.ascontiguousarray(a) for k, a in
+            (("x", x), ("v", v), ("f", f), ("fd", fd))}
+    L = np.zeros((n, 48), dtype=np.int32)
+    for i, l in enumerate(lists):
+        L[i, 0] = len(l)
+        L[i, 1:1 + len(l)] = np.asarray(l) + 1
+
+    def work(lo, hi):
+        for i in range(lo, hi):
+            a = ctypes.c_int(i + 1)
+            p = [ctypes.cast(rows[k].ctypes.data + 24 * i, dp) for k in ("x", "v", "f", "fd")]
+            mm = ctypes.c_double(m[i])
+            tt = ctypes.c_double(t0[i])
+            lib.irr_simd_set_jp_(ctypes.byref(a), *p, ctypes.byref(mm), ctypes.byref(tt))
+            lib.irr_simd_set_list_(ctypes.byref(a), ctypes.cast(
+                L.ctypes.data + 4 * 48 * i, ctypes.POINTER(ctypes.c_int)))
+
+    gpunb.irr_open(n, 48)
+    try:
+        with ThreadPoolExecutor(8) as ex:
+            list(ex.map(lambda k: work(k * n // 8, (k + 1) * n // 8), range(8)))
+        gpunb._check()
+        idx = np.arange(n)
+        got = gpunb.irr_firr(0.001, idx + 1)
+    finally:
+        gpunb.irr_close()
+    want = oracle.irr(0.001, idx, t0, x, v, f, fd, m, lists)
+    check(got, want, scales(0.001, idx, t0, x, v, f, fd, m, lists))
