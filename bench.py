@@ -225,10 +225,14 @@ def main():
 
     call(0)
     gpunb._check()
-    t2 = time.perf_counter()
-    for q in range(nchunk):
-        call(q)
-    abi_rate = nchunk * ci * n / (time.perf_counter() - t2)
+    # Host-latency bound (one synchronising call per 1024 i): best of three
+    # passes of 32 calls, so a busy host core does not set the figure.
+    abi_rate = 0.0
+    for _ in range(3):
+        t2 = time.perf_counter()
+        for q in range(nchunk):
+            call(q)
+        abi_rate = max(abi_rate, nchunk * ci * n / (time.perf_counter() - t2))
     gpunb._check()
     gpunb.gpunb_close()
 
