@@ -176,3 +176,43 @@ def test_device_entry_point_matches_abi():
             assert np.array_equal(L[O[i]:O[i] + k], ref[3][i, 1:1 + k])
     finally:
         gpunb.gpunb_close()
+
+
+def test_king_131072_invariants_all_particles():
+    """Size-independent properties over all 131072 i of the full sweep.
+
+    - h2 = 0 (no neighbours): Newton's third law for acceleration and jerk,
+      sum_i m_i a_i = 0 and sum_i m_i jerk_i = 0, to the FP32 pair-arithmetic
+      level, and the regular-force potential equals gpupot's for every i.
+    - Uniform h2 and dtr: the neighbour predicate min(r^2, r_p^2) < h2 is
+      symmetric in (i, j) (the FP64 differences change sign exactly), so the
+      lists are exactly symmetric: j lists i iff i lists j.
+    """
+    c = ics.king(131072, 6.0, seed=210, m=ics.kroupa(131072, seed=211))
+    n, lmax, nnbmax = c.n, 601, 550
+    gpunb.gpunb_open(n + 10)
+    try:
+        gpunb.gpunb_send(c.m, c.x, c.v)
+        acc, jrk, pot, lst = gpunb.gpunb_regf(np.zeros(n), np.zeros(n), c.x, c.v,
+                                              lmax, nnbmax, 0)
+        h = np.full(n, 0.05 ** 2)
+        d = np.full(n, 1.0 / 64)
+        _, _, _, lst2 = gpunb.gpunb_regf(h, d, c.x, c.v, lmax, nnbmax, 0)
+    finally:
+        gpunb.gpunb_close()
+    assert np.all(lst[:, 0] == 0)
+    m = c.m[:, None]
+    for name, a in (("acc", acc), ("jrk", jrk)):
+        tot = np.abs(np.sum(m * a, axis=0))
+        scale = np.sum(m * np.abs(a), axis=0)
+        assert np.all(tot <= 1e-6 * scale), (name, tot / scale)
+    phi = gpunb.gpupot(1, n, c.m, c.x)
+    assert np.max(np.abs(pot - phi) / phi) < FORCE_TOL
+    cnt = lst2[:, 0]
+    assert np.all((cnt >= 1) & (cnt <= nnbmax)), "no overflow expected at RS 0.05"
+    rows = np.repeat(np.arange(n), cnt)
+    cols = np.concatenate([lst2[i, 1:1 + cnt[i]] for i in range(n)])
+    off = rows != cols
+    a = rows[off].astype(np.int64) * n + cols[off]
+    b = cols[off].astype(np.int64) * n + rows[off]
+    assert np.array_equal(np.sort(a), np.sort(b)), "lists not symmetric"
