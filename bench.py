@@ -333,7 +333,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f64" if args.prec >= 64 else "f32",
         "data": "synthetic",
         "config": {
             "workload": "King W0=6, N=131072, Kroupa IMF 0.08-150 Msun; one "
@@ -341,8 +341,10 @@ def main():
                         "regular force of N i vs N j), inputs resident in HBM",
             "n": n, "ni_per_step": n, "pairs_per_step": pairs_per_step,
             "nnb_mean": nnb_mean, "nnbmax": nnbmax,
-            "precision": "FP64 differences, FP64 neighbour predicate and "
-                         "accumulation; FP32 pair arithmetic",
+            "precision": ("FP64 pair arithmetic (v_rsq_f64, %d Newton steps)"
+                          % (args.prec - 64)) if args.prec >= 64 else
+                         ("FP64 differences, FP64 neighbour predicate and "
+                          "accumulation; FP32 pair arithmetic"),
         },
         "roofline": {
             "bound": "mfma",

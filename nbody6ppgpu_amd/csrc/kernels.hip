@@ -271,14 +271,16 @@ __global__ void k_zero_int(int *p, int n) {
 // scalar cache as SGPR operands: no LDS and no VGPRs are spent on them.
 //
 // Per pair (fast path): FP64 differences dx = x_j - x_i, dv = v_j - v_i
-// (exactly the oracle's), rounded to FP32; FP32 r2, dx.dv, |dv|^2, the
-// predicted distance r2p = r2 + 2 dtr dx.dv + dtr^2 |dv|^2, one v_rsq_f32
-// and the force sums.  The neighbour test first compares the FP32
-// min(r2, r2p) with a per-lane threshold that is provably above the FP64
-// threshold by more than the FP32 evaluation error; only lanes below it take
-// the FP64 path, which evaluates the oracle's predicate bit for bit.  Pairs
-// that are no neighbours therefore never pay for FP64 arithmetic beyond the
-// six differences, and the lists are exactly the FP64 lists.
+// (exactly the oracle's), rounded to FP32; FP32 r2, dx.dv, the screen value
+// q = r2 + 2 dtr dx.dv (a lower bound of the predicted distance r2p = q +
+// dtr^2 |dv|^2), one v_rsq_f32 and the force sums.  The neighbour test first
+// compares the FP32 min(r2, q) with a per-lane threshold that is provably
+// above the FP64 threshold by more than the FP32 evaluation error; only
+// lanes below it take the FP64 path, which evaluates the oracle's predicate
+// bit for bit.  Pairs that are no neighbours therefore never pay for FP64
+// arithmetic beyond the six differences, and the lists are exactly the FP64
+// lists.  The j records of the next group are loaded while the current
+// group is evaluated.
 //
 // The FP32 partial sums are folded into FP64 accumulators every kTile pairs.
 // ---------------------------------------------------------------------------
@@ -321,7 +323,7 @@ __global__ __launch_bounds__(64) void k_regf_pass1(
 
     double x0 = 0.0, y0 = 0.0, z0 = 0.0, u0 = 0.0, w0 = 0.0, q0 = 0.0;
     double h2d = 0.0, dtrd = 0.0;
-    float dtr2x = 0.0f, dtrsq = 0.0f;   // 2 dtr and dtr^2 (FP32)
+    float dtr2x = 0.0f;   // 2 dtr (FP32)
     float thr = -1.0f;  // inactive lanes never enter the FP64 path
     if (act) {
         x0 = xi[3 * (size_t)i + 0];
@@ -336,19 +338,22 @@ __global__ __launch_bounds__(64) void k_regf_pass1(
         if (chk != chk) atomicOr(err, ERR_NAN_I);
         const float dtr32 = (float)dtrd;
         dtr2x = 2.0f * dtr32;
-        dtrsq = dtr32 * dtr32;
-        // FP32 screening threshold.  The fast path evaluates
-        //   r2p = fma(dtr^2, |dv|^2, fma(2 dtr, dx.dv, r2))
-        // from the FP32-rounded differences.  Every operand and operation
-        // contributes a relative error of at most u to one of the three
-        // nonnegative magnitudes r2, 2|dtr| sum|dx_c dv_c|, dtr^2 |dv|^2, so
-        // |r2p_32 - r2p| <= 10u S with S = sum_c (|dx_c| + |dtr| |dv_c|)^2
-        // <= (|dx| + |dtr| |dv|)^2.  If r2p < H (H the FP64 threshold: h2, or
-        // m_j h2 >= m_min h2), then |dx| <= sqrt(H) + |dtr| |dv| and, with
+        // FP32 screening threshold.  The list predicate is
+        // min(r2, r2p) < H with r2p = |dx + dtr dv|^2 = q + dtr^2 |dv|^2 and
+        // q = r2 + 2 dtr dx.dv; since dtr^2 |dv|^2 >= 0, min(r2, q) <= min(r2,
+        // r2p), so the screen value min(r2, q) flags a superset of the
+        // neighbours (5% more pairs on a King W0=6 model) without |dv|^2.
+        // The fast path evaluates q = fma(2 dtr, dx.dv, r2) from the
+        // FP32-rounded differences; every operand and operation contributes
+        // a relative error of at most u to one of the nonnegative magnitudes
+        // r2, 2|dtr| sum|dx_c dv_c|, so |q_32 - q| <= 7.2u S with
+        // S = sum_c (|dx_c| + |dtr| |dv_c|)^2 <= (|dx| + |dtr| |dv|)^2.  If
+        // r2p < H (H the FP64 threshold: h2, or m_j h2 >= m_min h2), then
+        // |dx| <= sqrt(H) + |dtr| |dv| and, with
         // rho = |dtr| (|v_i| + max|v_j|) / sqrt(H), S <= H (1 + 2 rho)^2.
-        // r2 itself is within a relative 5.1u.  The threshold doubles the
-        // r2p margin and adds 16u (24u with the mass factor) for the
-        // rounding of the FP64 reference values and of the threshold.
+        // r2 itself is within a relative 5.1u.  The threshold allows 20u S
+        // and adds 16u (24u with the mass factor) for the rounding of the
+        // FP64 reference values and of the threshold.
         if (h2d > 0.0) {
             const double vmax = bitsd(jst->vmax_bits);
             double hmin = h2d;
@@ -421,6 +426,13 @@ __global__ __launch_bounds__(64) void k_regf_pass1(
         }
     };
 
+    // The records of the next group are loaded while the current group is
+    // evaluated, hiding the scalar-load latency inside the wave (5% faster
+    // per launch).  The j array carries 64 spare records past its last tile,
+    // so the look-ahead of the last group stays in bounds.
+    JRec Jn[G];
+#pragma unroll
+    for (int k = 0; k < G; ++k) Jn[k] = jp[tb * kTile + k];
     for (int t = tb; t < te; ++t) {
         const int jt = t * kTile;
         for (int jg = 0; jg < kTile; jg += G) {
@@ -428,7 +440,9 @@ __global__ __launch_bounds__(64) void k_regf_pass1(
             bool fl[G];
             bool any = false;
 #pragma unroll
-            for (int k = 0; k < G; ++k) J[k] = jp[jt + jg + k];
+            for (int k = 0; k < G; ++k) J[k] = Jn[k];
+#pragma unroll
+            for (int k = 0; k < G; ++k) Jn[k] = jp[jt + jg + G + k];
 #pragma unroll
             for (int k = 0; k < G; ++k) {
                 const double dx = J[k].x - x0;
@@ -440,11 +454,10 @@ __global__ __launch_bounds__(64) void k_regf_pass1(
                 const float gz = (float)(J[k].vz - q0);
                 const float r2 = fmaf(fz, fz, fmaf(fy, fy, fx * fx));
                 const float rv = fmaf(fz, gz, fmaf(fy, gy, fx * gx));
-                const float v2 = fmaf(gz, gz, fmaf(gy, gy, gx * gx));
-                const float r2p = fmaf(dtrsq, v2, fmaf(dtr2x, rv, r2));
+                const float q = fmaf(dtr2x, rv, r2);
                 float tj = thr;
                 if (MFLAG) tj = fmaxf(thr * J[k].m32, FLT_MIN);
-                fl[k] = fminf(r2, r2p) < tj;
+                fl[k] = fminf(r2, q) < tj;
                 any = any || fl[k];
                 const float rinv = fl[k] ? 0.0f : __builtin_amdgcn_rsqf(r2);
                 pair_terms(J[k], fx, fy, fz, gx, gy, gz, r2, rv, rinv);
@@ -484,12 +497,13 @@ __global__ __launch_bounds__(64) void k_regf_pass1(
 // at the rate of an unpacked FP32 FMA, so the whole pair can be evaluated in
 // FP64 for about the instruction count of the FP32 form: no conversions and
 // no FP32 -> FP64 folds (the sums accumulate in FP64 directly).  The screen
-// uses FMA-based FP64 r2 and r2p with a band of a few 2^-53 around the
-// threshold; pairs inside it are decided by the oracle's exact predicate as in
-// the FP32 kernel.  1/r: v_rsq_f64 refined by NR Newton steps.  Measured on
-// MI355X this form runs 2% (NR = 0) to 11% (NR = 1) slower than the FP32
-// kernel: FP64 FMAs issue at full rate but v_rsq_f64 and the FP64 operand
-// traffic do not.
+// is the FP32 kernel's min(r2, r2 + 2 dtr dx.dv) in FP64 with a band of a
+// few 2^-53 around the threshold; pairs inside it are decided by the
+// oracle's exact predicate as in the FP32 kernel.  1/r: v_rsq_f64 refined by
+// NR Newton steps.  Measured on MI355X (King W0=6, N = 131072, one regular
+// block) this form takes 1.45x the FP32 kernel's time at NR = 0: FP64 FMAs
+// issue at the rate of unpacked FP32 ones, but FP64 operands occupy register
+// pairs and the FP32 kernel's mixed FP32/FP64/convert stream co-issues better.
 // Jerk is accumulated as J1 = sum mr3 dv and J2 = sum mr3 (dx.dv/r2) dx and
 // combined as J1 - 3 J2 at the end (one multiply per pair less).
 // ---------------------------------------------------------------------------
@@ -531,7 +545,7 @@ __global__ __launch_bounds__(64) void k_regf_pass1_f64(
     const int jbeg = tb * kTile;
 
     double x0 = 0.0, y0 = 0.0, z0 = 0.0, u0 = 0.0, w0 = 0.0, q0 = 0.0;
-    double h2d = 0.0, dtrd = 0.0;
+    double h2d = 0.0, dtrd = 0.0, dtr2x = 0.0;   // dtr2x = 2 dtr
     double thr = -1.0;
     if (act) {
         x0 = xi[3 * (size_t)i + 0];
@@ -542,9 +556,14 @@ __global__ __launch_bounds__(64) void k_regf_pass1_f64(
         q0 = vi[3 * (size_t)i + 2];
         h2d = h2[i];
         dtrd = dtr[i];
+        dtr2x = 2.0 * dtrd;
         const double chk = x0 + y0 + z0 + u0 + w0 + q0 + h2d + dtrd;
         if (chk != chk) atomicOr(err, ERR_NAN_I);
-        // Same bound as the FP32 screen with the FP64 unit roundoff.
+        // The FP32 screen's bound with the FP64 unit roundoff: |q_64 - q| <=
+        // 4.1u S <= 4.1u H (1 + 2 rho)^2 for the screen value, and the
+        // oracle's r2p (separate multiply and add in the prediction) within
+        // 3.1u H + 4.1u H (1 + 2 rho) of the exact one near the threshold;
+        // the threshold allows twice their sum.
         if (h2d > 0.0) {
             const double vmax = bitsd(jst->vmax_bits);
             double hmin = h2d;
@@ -555,7 +574,8 @@ __global__ __launch_bounds__(64) void k_regf_pass1_f64(
             const double u = 1.1102230246251565e-16;  // 2^-53
             const double rho =
                 fabs(dtrd) * (speed_up(u0, w0, q0) + vmax) / sqrt(hmin);
-            const double eps = 2.0 * (8.0 * u * (1.0 + 2.0 * rho) + 4.0 * u) +
+            const double g = 1.0 + 2.0 * rho;
+            const double eps = 2.0 * (8.0 * u * g * g + 4.0 * u) +
                                (MFLAG ? 24.0 : 16.0) * u;
             const double tt = h2d + h2d * eps;
             thr = fmax(nextafter(nextafter(tt, INFINITY), INFINITY), 1e-30);
@@ -583,8 +603,8 @@ __global__ __launch_bounds__(64) void k_regf_pass1_f64(
     };
 
     auto pair_terms = [&](const JRec &J, double dx, double dy, double dz,
-                          double dvx, double dvy, double dvz, double rinv) {
-        const double rv = fma(dz, dvz, fma(dy, dvy, dx * dvx));
+                          double dvx, double dvy, double dvz, double rv,
+                          double rinv) {
         const double rinv2 = rinv * rinv;
         const double mr = J.m * rinv;
         const double mr3 = mr * rinv2;
@@ -616,10 +636,18 @@ __global__ __launch_bounds__(64) void k_regf_pass1_f64(
             ++c;
         } else if (r2d != 0.0) {
             // Rare (band pairs, r2 < 1e-30): exact FP64 1/sqrt.
-            pair_terms(J, dx, dy, dz, dvx, dvy, dvz, 1.0 / sqrt(r2d));
+            const double rv = fma(dz, dvz, fma(dy, dvy, dx * dvx));
+            pair_terms(J, dx, dy, dz, dvx, dvy, dvz, rv, 1.0 / sqrt(r2d));
         }
     };
 
+    // The records of the next group are loaded while the current group is
+    // evaluated, hiding the scalar-load latency inside the wave (5% faster
+    // per launch).  The j array carries 64 spare records past its last tile,
+    // so the look-ahead of the last group stays in bounds.
+    JRec Jn[G];
+#pragma unroll
+    for (int k = 0; k < G; ++k) Jn[k] = jp[tb * kTile + k];
     for (int t = tb; t < te; ++t) {
         const int jt = t * kTile;
         for (int jg = 0; jg < kTile; jg += G) {
@@ -627,7 +655,9 @@ __global__ __launch_bounds__(64) void k_regf_pass1_f64(
             bool fl[G];
             bool any = false;
 #pragma unroll
-            for (int k = 0; k < G; ++k) J[k] = jp[jt + jg + k];
+            for (int k = 0; k < G; ++k) J[k] = Jn[k];
+#pragma unroll
+            for (int k = 0; k < G; ++k) Jn[k] = jp[jt + jg + G + k];
 #pragma unroll
             for (int k = 0; k < G; ++k) {
                 const double dx = J[k].x - x0;
@@ -637,16 +667,14 @@ __global__ __launch_bounds__(64) void k_regf_pass1_f64(
                 const double dvy = J[k].vy - w0;
                 const double dvz = J[k].vz - q0;
                 const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-                const double px = fma(dtrd, dvx, dx);
-                const double py = fma(dtrd, dvy, dy);
-                const double pz = fma(dtrd, dvz, dz);
-                const double r2p = fma(pz, pz, fma(py, py, px * px));
+                const double rv = fma(dz, dvz, fma(dy, dvy, dx * dvx));
+                const double q = fma(dtr2x, rv, r2);
                 double tj = thr;
                 if (MFLAG) tj = fmax(thr * J[k].m, 1e-30);
-                fl[k] = fmin(r2, r2p) < tj;
+                fl[k] = fmin(r2, q) < tj;
                 any = any || fl[k];
                 const double rinv = fl[k] ? 0.0 : rsq64(r2);
-                pair_terms(J[k], dx, dy, dz, dvx, dvy, dvz, rinv);
+                pair_terms(J[k], dx, dy, dz, dvx, dvy, dvz, rv, rinv);
             }
             if (any) {
 #pragma unroll
