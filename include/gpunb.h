@@ -303,6 +303,75 @@ int gpunb_timing(int enable);
 int gpunb_timing_read(double *ms_total, long long *launches,
                       long long *pairs);
 
+/* ------------------------------------------------------------------------ */
+/* Regular corrector on the device.  Replaces the host loop                  */
+/*   DO II = 1, NI: CALL REGCOR_GPU(I, x_i, v_i, GPU_A, GPU_JERK, LISTGP)     */
+/* of util_gpu.F:152-180 (REGCOR_GPU: regcor_gpu.F:1-805) for an isolated    */
+/* cluster: KZ(13) = KZ(14) = 0 (no cloud or external force; the KZ(14) <= 2 */
+/* tidal FIDOT term uses tidal4), KZ(37) = KZ(38) = 0 (no CHECKL / FPCORR).  */
+/* ------------------------------------------------------------------------ */
+
+/* Scalars of the reference's common6.h that REGCOR_GPU reads. */
+typedef struct {
+    double time;       /* TIME */
+    double rscale;     /* RSCALE */
+    double rdens[3];   /* RDENS: density centre */
+    double rc2;        /* RC2: core radius squared */
+    double rsfac;      /* RSFAC (adjust.F:220) */
+    double znbmin;     /* ZNBMIN (input.F:122) */
+    double smax;       /* SMAX */
+    double dtmin;      /* DTMIN */
+    double etar;       /* ETAR */
+    double smin;       /* SMIN */
+    double bodym;      /* BODYM: mean mass */
+    double tidal4;     /* TIDAL(4); 0 for an isolated cluster */
+    int nnbopt;        /* NNBOPT */
+    int nnbmax;        /* NNBMAX */
+    int n;             /* N: single bodies are 1..N, c.m. bodies N+1..NTOT */
+    int ntot;          /* NTOT */
+    int ifirst;        /* IFIRST: j-particle 0 of the library is body IFIRST */
+    int lmax;          /* LMAX: row length of LIST (>= nnbmax + 3) */
+    int nb_flag;       /* NB_FLAG (KZ(39) = 0 or 2): radii about RDENS */
+    int m_flag;        /* M_FLAG (KZ(39) = 2 or 3): mass-weighted lists */
+} gpunb_regcor_par;
+
+/* Device arrays of the body state, indexed by body I - 1 (vectors as three
+ * consecutive doubles, the Fortran (3,NMAX) layout; LIST rows of lmax ints:
+ * count, then the 1-based members in ascending order). */
+typedef struct {
+    const double *x0, *x0dot;            /* X0, X0DOT (read) */
+    double *xn, *xndot;                  /* XN, XNDOT (written) */
+    double *fi, *fidot, *fr, *frdot;     /* FI, FIDOT, FR, FRDOT */
+    double *d0, *d0r, *d1r, *d2r, *d3r;  /* D0, D0R, D1R, D2R, D3R (written) */
+    double *t0r, *stepr, *step;          /* T0R, STEPR, STEP */
+    const double *t0;                    /* T0 (read) */
+    double *timenw, *rs;                 /* TIMENW, RS */
+    const double *body;                  /* BODY (read) */
+    const double *h;                     /* H(NPAIRS) for I > N; NULL if none */
+    int *list;                           /* LIST(LMAX,NMAX) */
+} gpunb_regcor_state;
+
+/* Correct the ni bodies d_idx[] (1-based I, distinct) at par->time, after
+ * gpunb_regf_dev at the same time, on the library stream (not synchronised):
+ *   d_xi, d_vi       : double[ni][3] predicted x_i, v_i (the regf inputs)
+ *   d_freg, d_fdr    : double[ni][3] regular force / derivative from regf
+ *                      (updated when a small-step neighbour is retained)
+ *   d_nnb, d_off, d_list : the compact lists of gpunb_regf_dev (0-based j,
+ *                      body itself included; overflows must be resolved)
+ *   list_len         : entries of d_list that may be read (a body whose
+ *                      list reaches past them is reported as invalid)
+ *   d_counters       : int[4] accumulating NBVOID, NBSMIN, NBFLUX, NICONV
+ * Neighbour positions, velocities and masses are the library's current
+ * (predicted) j records: body J is j-particle J - IFIRST.  STEP(J) of
+ * another body is read as it was before the call.  Returns 0, or -1 with
+ * gpunb_last_error() set; invalid bodies (index, duplicate, list range or
+ * order) are left unchanged and reported by gpunb_sync(). */
+int gpunb_regcor_dev(const gpunb_regcor_par *par,
+                     const gpunb_regcor_state *st, int ni, const int *d_idx,
+                     const double *d_xi, const double *d_vi, double *d_freg,
+                     double *d_fdr, const int *d_nnb, const int *d_off,
+                     const int *d_list, long long list_len, int *d_counters);
+
 #ifdef __cplusplus
 }
 #endif

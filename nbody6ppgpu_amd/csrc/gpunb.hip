@@ -139,6 +139,11 @@ struct State {
     DevBuf<double> d_pin, d_ppart, d_pout;
     HostBuf<double> h_pin, h_pout;
 
+    // Regular corrector scratch (gpunb_regcor_dev); rc_mark stays zero
+    // between calls.
+    DevBuf<int> rc_nlist, rc_jj, rc_mark;
+    DevBuf<double> rc_step, rc_tnw;
+
     double time_send = 0, time_grav = 0, time_nb = 0, time_out = 0;
     long long numInter = 0;
     int icall = 0, ini = 0, isend = 0;
@@ -252,6 +257,10 @@ void err_check() {
     if (e & ERR_NAN_J) m += "NaN in j-particle data; ";
     if (e & ERR_NAN_I) m += "NaN in i-particle data; ";
     if (e & ERR_LIST_CAP) m += "neighbour list buffer too small; ";
+    if (e & ERR_REGCOR)
+        m += "regular corrector: body index out of range or listed twice, or "
+             "a list too long, unsorted or outside the j set (those bodies "
+             "are left unchanged); ";
     throw Error(m);
 }
 
@@ -501,6 +510,11 @@ void do_close() {
     S.d_iout.release();
     S.h_iout.release();
     S.d_list.release();
+    S.rc_nlist.release();
+    S.rc_jj.release();
+    S.rc_mark.release();
+    S.rc_step.release();
+    S.rc_tnw.release();
     S.h_list.release();
     S.nbmax = 0;
     S.nj = 0;
@@ -1117,6 +1131,77 @@ GPUNB_EXPORT int gpunb_regf_dev(int ni, const double *d_h2, const double *d_dtr,
         S.icall++;
         enqueue_regf(ni, d_xi, d_vi, d_h2, d_dtr, d_acc, d_jrk, d_pot, nnbmax,
                      m_flag, d_nnb, d_nnid, d_list, d_off, d_total, list_cap);
+    });
+}
+
+GPUNB_EXPORT int gpunb_regcor_dev(const gpunb_regcor_par *par,
+                                  const gpunb_regcor_state *st, int ni,
+                                  const int *d_idx, const double *d_xi,
+                                  const double *d_vi, double *d_freg,
+                                  double *d_fdr, const int *d_nnb,
+                                  const int *d_off, const int *d_list,
+                                  long long list_len, int *d_counters) {
+    return guarded("gpunb_regcor_dev", [&] {
+        require_open("gpunb_regcor_dev");
+        REQUIRE(par && st, "gpunb_regcor_dev: par and st must not be NULL");
+        REQUIRE(ni >= 0, "gpunb_regcor_dev: ni must be >= 0");
+        if (ni == 0) return;
+        const gpunb_regcor_par &P = *par;
+        REQUIRE(P.ntot >= 1 && P.n >= 0 && P.n <= P.ntot,
+                "gpunb_regcor_dev: need 0 <= n <= ntot, ntot >= 1");
+        REQUIRE(P.ifirst >= 1 && P.ifirst <= P.ntot,
+                "gpunb_regcor_dev: need 1 <= ifirst <= ntot");
+        REQUIRE(P.ntot - P.ifirst + 1 <= S.nj,
+                "gpunb_regcor_dev: bodies ifirst..ntot must all be j-particles "
+                "(send or jpset and predict them first)");
+        REQUIRE(P.nnbmax >= 0 && P.lmax >= P.nnbmax + 3,
+                "gpunb_regcor_dev: need lmax >= nnbmax + 3");
+        REQUIRE((P.nb_flag == 0 || P.nb_flag == 1) &&
+                    (P.m_flag == 0 || P.m_flag == 1),
+                "gpunb_regcor_dev: nb_flag and m_flag must be 0 or 1");
+        const gpunb_regcor_state &T = *st;
+        REQUIRE(T.x0 && T.x0dot && T.xn && T.xndot && T.fi && T.fidot &&
+                    T.fr && T.frdot && T.d0 && T.d0r && T.d1r && T.d2r &&
+                    T.d3r && T.t0r && T.stepr && T.step && T.t0 &&
+                    T.timenw && T.rs && T.body && T.list,
+                "gpunb_regcor_dev: a state array is NULL");
+        REQUIRE(d_idx && d_xi && d_vi && d_freg && d_fdr && d_nnb && d_off &&
+                    d_list && d_counters,
+                "gpunb_regcor_dev: a block array is NULL");
+        REQUIRE(list_len >= 0, "gpunb_regcor_dev: list_len must be >= 0");
+        set_device();
+        S.rc_nlist.ensure((size_t)ni * P.lmax);
+        S.rc_jj.ensure((size_t)ni * 2 * P.lmax);
+        S.rc_step.ensure((size_t)ni);
+        S.rc_tnw.ensure((size_t)ni);
+        if (S.rc_mark.n < (size_t)P.ntot) {
+            S.rc_mark.ensure((size_t)P.ntot);
+            HIPCHK(hipMemsetAsync(S.rc_mark.p, 0, S.rc_mark.n * sizeof(int),
+                                  S.st));
+        }
+        RegcorLaunch L;
+        L.par = P;
+        L.st = T;
+        L.ni = ni;
+        L.idx = d_idx;
+        L.xi = d_xi;
+        L.vi = d_vi;
+        L.freg = d_freg;
+        L.fdr = d_fdr;
+        L.nnb = d_nnb;
+        L.off = d_off;
+        L.lst = d_list;
+        L.list_len = list_len;
+        L.jp = S.jp.p;
+        L.nj = S.nj;
+        L.nlist = S.rc_nlist.p;
+        L.jj = S.rc_jj.p;
+        L.step_new = S.rc_step.p;
+        L.tnw_new = S.rc_tnw.p;
+        L.mark = S.rc_mark.p;
+        L.cnt = d_counters;
+        L.err = S.err.p;
+        launch_regcor(S.st, L);
     });
 }
 

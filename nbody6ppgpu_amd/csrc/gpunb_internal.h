@@ -6,6 +6,8 @@
 #include <cstdint>
 #include <functional>
 
+#include "gpunb.h"
+
 namespace gpunb {
 
 // One j-particle in HBM: 64 bytes, so a wavefront fetches a whole record with
@@ -44,6 +46,7 @@ enum : unsigned {
     ERR_NAN_J = 1u,      // NaN in j data (send / jpset)
     ERR_NAN_I = 2u,      // NaN in i data (regf)
     ERR_LIST_CAP = 4u,   // compact list buffer too small (device entry point)
+    ERR_REGCOR = 8u,     // regular corrector: invalid body index or list
 };
 
 // Far-away parking place of the padding j records (mass 0).
@@ -126,6 +129,27 @@ void launch_pot(hipStream_t st, const PRec *p, int n, int ntile, int i0,
                 int ni, int nslice, double *part, double *pot);
 void launch_pack_p(hipStream_t st, const double *m, const double *x, int n,
                    int n_pad, PRec *p, unsigned *err);
+
+// One regular-corrector call (regcor.hip).
+struct RegcorLaunch {
+    gpunb_regcor_par par;
+    gpunb_regcor_state st;
+    int ni;
+    const int *idx;
+    const double *xi, *vi;
+    double *freg, *fdr;
+    const int *nnb, *off, *lst;   // compact lists of the regular force
+    long long list_len;           // readable entries of lst
+    const JRec *jp;               // predicted j records
+    int nj;
+    int *nlist;                   // scratch [ni][lmax]: NLIST
+    int *jj;                      // scratch [ni][2 lmax]: JJLIST
+    double *step_new, *tnw_new;   // scratch [ni]
+    int *mark;                    // [ntot], zero between calls
+    int *cnt;                     // [4] counters
+    unsigned *err;
+};
+void launch_regcor(hipStream_t st, const RegcorLaunch &L);
 
 // Occupancy-derived resident-wave capacity of the regular-force kernel.
 int regf_resident_waves(int m_flag, int group, int prec);

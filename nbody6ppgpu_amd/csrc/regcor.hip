@@ -1,0 +1,490 @@
+// regcor.hip -- the regular corrector of a block of bodies on the device:
+// the work NBODY6++GPU's host does per body after gpunb_regf
+// (src/Main/regcor_gpu.F:1-805, called from util_gpu.F:152-180), for an
+// isolated cluster (no cloud or external force, no CHECKL / FPCORR):
+//   * the Ahmad-Cohen neighbour-radius update (regcor_gpu.F:146-253),
+//   * the old/new list comparison with the small-step retention
+//     (:266-370) and the irregular-force change of the lost and gained
+//     neighbours (:400-438),
+//   * the Hermite corrector of the regular force and the new D0..D3R
+//     (:440-551), the list copy (:605-613) and the new regular / irregular
+//     step (:615-805).
+//
+// One thread per body: the work per body is a few hundred FP64 operations
+// with data-dependent branches and list walks, far too little for a
+// wavefront.  Neighbour positions, velocities and masses come from the
+// library's predicted j records (the X, XDOT, BODY the host would read).
+//
+// The arithmetic is the reference's, operation for operation and in its
+// evaluation order (the library is built with -ffp-contract=off), so the
+// results match the CPU restatement bit for bit where the math library's
+// sqrt / pow agree; REAL*4 literals and FLOAT() of the Fortran are float
+// values here as there.  STEP(J) of other bodies is read as it was before
+// the block: the new STEP / TIMENW are written by a second launch.
+
+#include "gpunb_internal.h"
+
+namespace gpunb {
+
+namespace {
+
+__device__ inline double sqd(double a) { return a * a; }
+
+constexpr double kOne3 = 1.0 / 3.0;    // ONE3 (zero.f:113)
+constexpr double kOne12 = 1.0 / 12.0;  // ONE12 (zero.f:116)
+constexpr int kMaxHalve = 2200;        // > 2098 halvings: DBL_MAX -> 0
+
+// Pair force / derivative of neighbour record R on (xi, xidot), added
+// (sign > 0) to or subtracted from (f, fd) as regcor_gpu.F:349-357 and
+// :409-416 form them.
+__device__ inline void pair_add(const JRec &R, const double *xi,
+                                const double *xidot, bool add, double *f,
+                                double *fd) {
+    const double a1 = R.x - xi[0];
+    const double a2 = R.y - xi[1];
+    const double a3 = R.z - xi[2];
+    const double dv0 = R.vx - xidot[0];
+    const double dv1 = R.vy - xidot[1];
+    const double dv2 = R.vz - xidot[2];
+    const double rij2 = a1 * a1 + a2 * a2 + a3 * a3;
+    const double dr2i = 1.0 / rij2;
+    const double dr3i = R.m * dr2i * sqrt(dr2i);
+    const double drdv = a1 * dv0 + a2 * dv1 + a3 * dv2;
+    const double drdp = 3.0 * drdv * dr2i;
+    const double a[3] = {a1, a2, a3};
+    const double dv[3] = {dv0, dv1, dv2};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (add) {
+            f[k] = f[k] + a[k] * dr3i;
+            fd[k] = fd[k] + (dv[k] - a[k] * drdp) * dr3i;
+        } else {
+            f[k] = f[k] - a[k] * dr3i;
+            fd[k] = fd[k] - (dv[k] - a[k] * drdp) * dr3i;
+        }
+    }
+}
+
+// Pre-pass: every body index in range and listed once.
+__global__ void k_regcor_mark(RegcorLaunch L) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= L.ni) return;
+    const int I = L.idx[q];
+    if (I < 1 || I > L.par.ntot) {
+        atomicOr(L.err, ERR_REGCOR);
+        return;
+    }
+    atomicAdd(&L.mark[I - 1], 1);
+}
+
+__global__ void k_regcor(RegcorLaunch L) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= L.ni) return;
+    const gpunb_regcor_par &P = L.par;
+    const gpunb_regcor_state &S = L.st;
+    const int I = L.idx[q];
+    if (I < 1 || I > P.ntot) return;            // flagged by the pre-pass
+    const int p = I - 1;
+    L.step_new[q] = S.step[p];
+    L.tnw_new[q] = S.timenw[p];
+    if (L.mark[p] != 1) {                       // listed twice: skip both
+        atomicOr(L.err, ERR_REGCOR);
+        return;
+    }
+    const int lmax = P.lmax;
+    int *LIST = S.list + (size_t)p * lmax;
+    int *nl = L.nlist + (size_t)q * lmax;
+    int *jj = L.jj + (size_t)q * 2 * lmax;
+#define NL(k) nl[(k) - 1]
+#define LI(k) LIST[(k) - 1]
+#define JJ(k) jj[(k) - 1]
+    auto rec = [&](int J) -> const JRec & { return L.jp[J - P.ifirst]; };
+    auto in_j = [&](int J) {
+        return J >= P.ifirst && J - P.ifirst < L.nj && J <= P.ntot;
+    };
+
+    // New list in util_gpu.F:103-110's form (1-based, the body dropped);
+    // both lists checked (members in the j set, strictly ascending) before
+    // anything is modified.  With sorted in-range lists every index of the
+    // list walks below stays inside its row.
+    const int cg = L.nnb[q];
+    const int og = L.off[q];
+    const int nnb0 = LI(1);
+    bool ok = cg >= 0 && og >= 0 && cg + 2 <= lmax &&
+              (long long)og + cg <= L.list_len && nnb0 >= 0 &&
+              nnb0 + 1 <= lmax && (I <= P.n || S.h != nullptr);
+    int nnb = 0;
+    if (ok) {
+        for (int k = 0; k < cg; ++k) {
+            const int J = L.lst[og + k] + P.ifirst;
+            if (J == I) continue;
+            if (!in_j(J) || (nnb > 0 && J <= NL(nnb + 1)) || nnb + 3 > lmax) {
+                ok = false;
+                break;
+            }
+            NL(nnb + 2) = J;
+            ++nnb;
+        }
+        NL(1) = nnb;
+        for (int l = 2; ok && l <= nnb0 + 1; ++l)
+            if (!in_j(LI(l)) || (l > 2 && LI(l) <= LI(l - 1))) ok = false;
+    }
+    if (!ok) {
+        atomicOr(L.err, ERR_REGCOR);
+        return;
+    }
+
+    double xi[3], xidot[3], freg[3], fdr[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        xi[k] = L.xi[3 * (size_t)q + k];
+        xidot[k] = L.vi[3 * (size_t)q + k];
+        freg[k] = L.freg[3 * (size_t)q + k];
+        fdr[k] = L.fdr[3 * (size_t)q + k];
+    }
+    double dfirr[3] = {0.0, 0.0, 0.0}, dfd[3] = {0.0, 0.0, 0.0};
+    const double dtr = P.time - S.t0r[p];
+    double ri2, rh2;
+    if (P.nb_flag == 1) {
+        ri2 = sqd(xi[0] - P.rdens[0]) + sqd(xi[1] - P.rdens[1]) +
+              sqd(xi[2] - P.rdens[2]);
+        rh2 = sqd(P.rscale);
+    } else {
+        ri2 = sqd(xi[0]) + sqd(xi[1]) + sqd(xi[2]);
+        rh2 = 9.0 * sqd(P.rscale);
+    }
+    double rs = S.rs[p];
+    const double rs2 = sqd(rs);
+    int c_void = 0, c_smin = 0, c_flux = 0, c_conv = 0;
+
+    if (nnb == 0) {
+        // No neighbours: a small mass at the centre (regcor_gpu.F:119-143);
+        // the radius update is skipped.
+        const double r2 = sqd(xi[0]) + sqd(xi[1]) + sqd(xi[2]);
+        const double fij = (double)0.01f * P.bodym / (r2 * sqrt(r2));
+        const double rdot =
+            3.0 * (xi[0] * xidot[0] + xi[1] * xidot[1] + xi[2] * xidot[2]) /
+            r2;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            dfirr[k] = -(fij * xi[k]);
+            dfd[k] = -((xidot[k] - rdot * xi[k]) * fij);
+        }
+        if (rdot > 0.0) {
+            const double a = 0.75 * rs, b = (double)0.1f * P.rscale;
+            rs = a > b ? a : b;
+        } else {
+            rs = (double)1.1f * rs;
+        }
+        c_void = 1;
+    } else {
+        // Neighbour radius (regcor_gpu.F:146-253).
+        float ft = 0.1f * ((float)P.nnbopt - (float)nnb);
+        ft = ft / (float)nnb;
+        const double fac = (double)(1.0f + ft);
+        double a3 = (double)(float)nnb * fac;
+        int nbave;
+        {
+            int mx = P.nnbmax - 50;
+            const int zi = (int)P.znbmin;
+            if (zi > mx) mx = zi;
+            nbave = P.nnbopt + 150 < mx ? P.nnbopt + 150 : mx;
+        }
+        {
+            const double cap = (double)(0.9f * (float)nbave);
+            if (cap < a3) a3 = cap;
+        }
+        const int nbp = (int)a3;
+        if (ri2 > rh2 && P.nb_flag == 1) {
+            double g = P.rscale / sqrt(ri2);
+            if (g < (double)0.1f) g = (double)0.1f;
+            a3 = a3 * g;
+        } else if (a3 < P.znbmin) {
+            a3 = P.znbmin;
+        }
+        double a4 = a3 / (double)(float)nnb;
+        if ((a3 - (double)(float)nnb0) * (a3 - (double)(float)nnb) < 0.0)
+            a4 = sqrt(a4);
+        const double ridot = (xi[0] - P.rdens[0]) * xidot[0] +
+                             (xi[1] - P.rdens[1]) * xidot[1] +
+                             (xi[2] - P.rdens[2]) * xidot[2];
+        if (ri2 > P.rc2 && P.nb_flag == 1 && ri2 < 100.0 * rh2)
+            a4 = a4 * (1.0 + ridot * dtr / ri2);
+        if (I > P.n && ri2 <= rh2) {
+            const double a2 = 100.0 * S.body[p] / fabs(S.h[I - P.n - 1]);
+            if (a2 > rs) {
+                const float qf = 1.0f - (float)nnb / (float)nbave;
+                a3 = (double)qf > 0.0 ? (double)qf : 0.0;
+                a4 = 1.0 + 0.5 * a3;
+            }
+        }
+        double a5 = P.rsfac * dtr;
+        if (0.25 < a5) a5 = 0.25;
+        a4 = a4 - 1.0;
+        if (a4 > a5)
+            a4 = a5;
+        else if (a4 < -a5)
+            a4 = -a5;
+        a3 = kOne3 * a4;
+        double a1 = 1.0 + a3 - a3 * a3;
+        if (nnb < P.znbmin && nbp > P.znbmin && a1 < 1.0) a1 = (double)1.05f;
+        if (fabs(a1 - 1.0) > (double)0.003f) {
+            if (sqd(rs) > ri2 && sqd(rs) < 2.0 * ri2 && nnb > 5) a1 = sqrt(a1);
+            rs = a1 * rs;
+        }
+        if (nnb > nbave && ridot < 0.0) {
+            const float drs =
+                ((float)nnb - (float)nbave) / (3.0f * (float)nnb);
+            rs = rs * (1.0 - (double)drs);
+        }
+        if (P.m_flag == 0 && nnb <= 3 && ri2 < 100.0 * rh2) {
+            double b1 = 2.0 * rs;
+            for (int l = 1; l <= nnb; ++l) {
+                const JRec &R = rec(NL(l + 1));
+                const double rij = sqrt(sqd(xi[0] - R.x) + sqd(xi[1] - R.y) +
+                                        sqd(xi[2] - R.z));
+                const double rsdot = ((xi[0] - R.x) * (xidot[0] - R.vx) +
+                                      (xi[1] - R.y) * (xidot[1] - R.vy) +
+                                      (xi[2] - R.z) * (xidot[2] - R.vz)) /
+                                     rij;
+                const double c = rij + rsdot * dtr;
+                if (c < b1) b1 = c;
+            }
+            const double c = (double)1.1f * rs;
+            rs = b1 > c ? b1 : c;
+        }
+    }
+
+    // Losses and gains (regcor_gpu.F:266-370): one pass over both sorted
+    // lists with the sentinel NTOT + 1 after each.
+    int nbloss = 0, nbgain = 0;
+    if (nnb0 == 0) {
+        nbgain = nnb;
+        for (int l = 1; l <= nnb; ++l) JJ(l) = NL(l + 1);
+    } else {
+        int jmin = 0;
+        int l = 2, lg = 2;
+        NL(nnb + 2) = P.ntot + 1;
+        NL(1) = LI(nnb0 + 1);
+        for (;;) {
+            if (LI(l) != NL(lg)) {
+                if (LI(l) >= NL(lg)) {
+                    ++nbgain;
+                    JJ(nnb0 + nbgain) = NL(lg);
+                    --l;
+                } else {
+                    ++nbloss;
+                    const int J = LI(l);
+                    JJ(nbloss) = J;
+                    if (S.step[J - 1] < P.smin) jmin = J;
+                    --lg;
+                }
+            }
+            if (l <= nnb0) {
+                ++l;
+                ++lg;
+            } else if (lg <= nnb) {
+                ++lg;
+                LI(l) = P.ntot + 1;
+            } else {
+                break;
+            }
+        }
+        if (jmin != 0) {
+            // Keep a lost neighbour with a small step inside 2 RS
+            // (regcor_gpu.F:328-370).
+            int k = 1;
+            for (;;) {
+                if (nnb > P.nnbmax || I > P.n) break;
+                const int J = JJ(k);
+                bool skip = S.step[J - 1] > P.smin || J < P.ifirst || J > P.n;
+                const JRec &R = rec(J);
+                if (!skip) {
+                    const double rij2 = sqd(xi[0] - R.x) + sqd(xi[1] - R.y) +
+                                        sqd(xi[2] - R.z);
+                    if (rij2 > 4.0 * rs2) skip = true;
+                }
+                if (!skip) {
+                    int ll = nnb + 1;
+                    for (;;) {
+                        if (NL(ll) < J) break;
+                        NL(ll + 1) = NL(ll);
+                        --ll;
+                        if (!(ll > 1)) break;
+                    }
+                    NL(ll + 1) = J;
+                    ++nnb;
+                    --nbloss;
+                    LI(nnb0 + 1) = NL(1);
+                    ++c_smin;
+                    pair_add(R, xi, xidot, false, freg, fdr);
+                    if (k > nbloss) break;
+                    for (int m = k; m <= nbloss; ++m) JJ(m) = JJ(m + 1);
+                    --k;
+                }
+                ++k;
+                if (!(k <= nbloss)) break;
+            }
+        }
+    }
+
+    // Irregular force change of the list change (regcor_gpu.F:400-438).
+    for (int l = 1; l <= nbloss; ++l)
+        pair_add(rec(JJ(l)), xi, xidot, false, dfirr, dfd);
+    for (int l = 1; l <= nbgain; ++l)
+        pair_add(rec(JJ(nnb0 + l)), xi, xidot, true, dfirr, dfd);
+
+    // Corrector (regcor_gpu.F:440-551).
+    const double dtsq = dtr * dtr;
+    const double dt6 = 6.0 / (dtr * dtsq);
+    const double dt2 = 2.0 / dtsq;
+    double dtsq12 = kOne12 * dtsq;
+    double dtr13 = kOne3 * dtr;
+    S.t0r[p] = P.time;
+    double stepi = S.step[p];
+    if (stepi < 5.0 * P.dtmin && dtr > 50.0 * stepi) {
+        dtr13 = 0.0;
+        dtsq12 = 0.0;
+    }
+    double fi[3], fid[3], d2r[3], d3r[3], xnd[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        fi[k] = S.fi[3 * (size_t)p + k];
+        fid[k] = S.fidot[3 * (size_t)p + k];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double dfr = S.fr[3 * (size_t)p + k] - freg[k] - dfirr[k];
+        const double fdr0 = fdr[k] + dfd[k];
+        const double frd = S.frdot[3 * (size_t)p + k];
+        const double sum = frd + fdr0;
+        const double at3 = 2.0 * dfr + dtr * sum;
+        const double bt2 = -3.0 * dfr - dtr * (sum + frd);
+        S.xn[3 * (size_t)p + k] =
+            S.x0[3 * (size_t)p + k] + ((double)0.6f * at3 + bt2) * dtsq12;
+        xnd[k] = S.x0dot[3 * (size_t)p + k] + (0.75 * at3 + bt2) * dtr13;
+        fi[k] = fi[k] + dfirr[k];
+        fid[k] = fid[k] + dfd[k];
+        if (k == 1) fid[0] = fid[0] - P.tidal4 * dfr;
+        if (k == 0) fid[1] = fid[1] + P.tidal4 * dfr;
+        d3r[k] = at3 * dt6;
+        d2r[k] = (3.0 * at3 + bt2) * dt2;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        S.xndot[3 * (size_t)p + k] = xnd[k];
+        S.fi[3 * (size_t)p + k] = fi[k];
+        S.fr[3 * (size_t)p + k] = freg[k];
+        S.fidot[3 * (size_t)p + k] = fid[k];
+        S.frdot[3 * (size_t)p + k] = fdr[k];
+        S.d0[3 * (size_t)p + k] = fi[k];
+        S.d0r[3 * (size_t)p + k] = freg[k];
+        S.d1r[3 * (size_t)p + k] = fdr[k];
+        S.d2r[3 * (size_t)p + k] = d2r[k];
+        S.d3r[3 * (size_t)p + k] = d3r[k];
+        L.freg[3 * (size_t)q + k] = freg[k];
+        L.fdr[3 * (size_t)q + k] = fdr[k];
+    }
+    S.rs[p] = rs;
+
+    // New list (regcor_gpu.F:605-613).
+    if (nbloss + nbgain > 0) {
+        LI(1) = nnb;
+        for (int l = 2; l <= nnb + 1; ++l) LI(l) = NL(l);
+        c_flux = nbloss + nbgain;
+    }
+
+    // New regular step (regcor_gpu.F:615-805).
+    const double fr2 = sqd(freg[0]) + sqd(freg[1]) + sqd(freg[2]);
+    const double w1 = sqd(fdr[0]) + sqd(fdr[1]) + sqd(fdr[2]);
+    const double w2 = sqd(d2r[0]) + sqd(d2r[1]) + sqd(d2r[2]);
+    const double w3 = sqd(d3r[0]) + sqd(d3r[1]) + sqd(d3r[2]);
+    double w0 = (sqrt(fr2 * w2) + w1) / (sqrt(w1 * w3) + w2);
+    w0 = P.etar * w0;
+    double ttmp = sqrt(w0);
+    const double dt0 = ttmp;
+    const double steprI = S.stepr[p];
+    if (ttmp > 2.0 * steprI) {
+        if (fmod(P.time, 2.0 * dtr) == 0.0) {
+            ttmp = 2.0 * dtr < P.smax ? 2.0 * dtr : P.smax;
+            for (int it = 0; it < kMaxHalve && dt0 > 2.0 * ttmp &&
+                             2.0 * ttmp < P.smax &&
+                             fmod(P.time, 2.0 * ttmp) == 0.0;
+                 ++it)
+                ttmp = 2.0 * ttmp;
+        } else {
+            ttmp = steprI;
+        }
+    } else if (ttmp < dtr) {
+        ttmp = 0.5 * dtr;
+        if (ttmp > dt0) ttmp = 0.5 * ttmp;
+    } else {
+        ttmp = steprI;
+    }
+    if (ttmp > (double)0.1f && ttmp > dtr) {
+        double dv2 = 0.0;
+        for (int k = 0; k < 3; ++k) dv2 = dv2 + sqd(xidot[k] - xnd[k]);
+        const double dtj =
+            dtr * pow(1.0e-04 * sqd(dtr) * fr2 / dv2, (double)0.1f);
+        if (dtj < ttmp) ttmp = dtr;
+    }
+    if (P.smax < ttmp) ttmp = P.smax;
+    S.stepr[p] = ttmp;
+    double tnw = S.timenw[p];
+    if (nnb0 == 0 && nnb >= 0) {
+        stepi = 0.125 * stepi;
+        tnw = S.t0[p] + stepi;
+    }
+    // Step halving (regcor_gpu.F:798-801), capped at kMaxHalve passes:
+    // enough to take any finite double to zero, so the cap only ends the
+    // loop where the reference's would not end (ttmp <= 0).
+    for (int it = 0; it < kMaxHalve && ttmp < stepi; ++it) {
+        stepi = 0.5 * stepi;
+        tnw = S.t0[p] + stepi;
+        ++c_conv;
+    }
+    L.step_new[q] = stepi;
+    L.tnw_new[q] = tnw;
+    if (c_void) atomicAdd(&L.cnt[0], c_void);
+    if (c_smin) atomicAdd(&L.cnt[1], c_smin);
+    if (c_flux) atomicAdd(&L.cnt[2], c_flux);
+    if (c_conv) atomicAdd(&L.cnt[3], c_conv);
+#undef NL
+#undef LI
+#undef JJ
+}
+
+// New STEP / TIMENW after every body is corrected; marks cleared.
+__global__ void k_regcor_commit(RegcorLaunch L) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= L.ni) return;
+    const int I = L.idx[q];
+    if (I < 1 || I > L.par.ntot) return;
+    const int p = I - 1;
+    if (L.mark[p] == 1) {
+        L.st.step[p] = L.step_new[q];
+        L.st.timenw[p] = L.tnw_new[q];
+    }
+}
+
+__global__ void k_regcor_unmark(RegcorLaunch L) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= L.ni) return;
+    const int I = L.idx[q];
+    if (I < 1 || I > L.par.ntot) return;
+    L.mark[I - 1] = 0;
+}
+
+}  // namespace
+
+void launch_regcor(hipStream_t st, const RegcorLaunch &L) {
+    if (L.ni <= 0) return;
+    const dim3 grid((unsigned)((L.ni + 255) / 256)), block(256);
+    hipLaunchKernelGGL(k_regcor_mark, grid, block, 0, st, L);
+    hipLaunchKernelGGL(k_regcor, dim3((unsigned)((L.ni + 63) / 64)), dim3(64),
+                       0, st, L);
+    hipLaunchKernelGGL(k_regcor_commit, grid, block, 0, st, L);
+    hipLaunchKernelGGL(k_regcor_unmark, grid, block, 0, st, L);
+}
+
+}  // namespace gpunb

@@ -57,6 +57,33 @@ def load(path: Optional[str] = None) -> ctypes.CDLL:
     return lib
 
 
+class RegcorPar(ctypes.Structure):
+    """gpunb_regcor_par: the common6.h scalars REGCOR_GPU reads."""
+    _fields_ = [("time", ctypes.c_double), ("rscale", ctypes.c_double),
+                ("rdens", ctypes.c_double * 3), ("rc2", ctypes.c_double),
+                ("rsfac", ctypes.c_double), ("znbmin", ctypes.c_double),
+                ("smax", ctypes.c_double), ("dtmin", ctypes.c_double),
+                ("etar", ctypes.c_double), ("smin", ctypes.c_double),
+                ("bodym", ctypes.c_double), ("tidal4", ctypes.c_double),
+                ("nnbopt", ctypes.c_int), ("nnbmax", ctypes.c_int),
+                ("n", ctypes.c_int), ("ntot", ctypes.c_int),
+                ("ifirst", ctypes.c_int), ("lmax", ctypes.c_int),
+                ("nb_flag", ctypes.c_int), ("m_flag", ctypes.c_int)]
+
+
+# gpunb_regcor_state fields, in order: per-body vectors, scalars, h, list.
+REGCOR_VEC = ("x0", "x0dot", "xn", "xndot", "fi", "fidot", "fr", "frdot",
+              "d0", "d0r", "d1r", "d2r", "d3r")
+REGCOR_SCAL = ("t0r", "stepr", "step", "t0", "timenw", "rs", "body")
+REGCOR_COUNTERS = ("nbvoid", "nbsmin", "nbflux", "niconv")
+
+
+class RegcorState(ctypes.Structure):
+    """gpunb_regcor_state: device pointers of the body state."""
+    _fields_ = ([(k, ctypes.c_void_p) for k in REGCOR_VEC + REGCOR_SCAL] +
+                [("h", ctypes.c_void_p), ("list", ctypes.c_void_p)])
+
+
 def _declare(lib: ctypes.CDLL) -> None:
     v = None
     sig = {
@@ -79,6 +106,11 @@ def _declare(lib: ctypes.CDLL) -> None:
                            [ctypes.c_int] + [ctypes.c_void_p] * 7 +
                            [ctypes.c_int, ctypes.c_int] +
                            [ctypes.c_void_p] * 5 + [ctypes.c_longlong]),
+        "gpunb_regcor_dev": (ctypes.c_int,
+                             [ctypes.POINTER(RegcorPar),
+                              ctypes.POINTER(RegcorState), ctypes.c_int] +
+                             [ctypes.c_void_p] * 8 +
+                             [ctypes.c_longlong, ctypes.c_void_p]),
         "gpunb_sync": (ctypes.c_int, []),
         "gpunb_stream": (ctypes.c_void_p, []),
         "gpunb_version": (ctypes.c_char_p, []),
