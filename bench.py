@@ -61,6 +61,127 @@ def load_traffic():
     return None
 
 
+def regcor_bench(c, h2, stepr, reps=5, cpu=True):
+    """Regular corrector on the device (gpunb_regcor_dev) for every body of
+    the King block, fed by gpunb_regf_dev lists at 0.9 RS (the old LIST) and
+    RS (the new lists), DTR = STEPR, TIME = 1: one call per rep on the same
+    input (the state is restored by a device copy outside the timed region),
+    HIP events on the library stream around the call.  The CPU restatement
+    of REGCOR_GPU (oracle/regcor.c, serial as util_gpu.F:152-180 calls it)
+    runs on the same block as the baseline.  Expects the engine open with
+    the bodies sent."""
+    from nbody6ppgpu_amd import gpunb, regcor
+    from nbody6ppgpu_amd.hiprt import DeviceArray, Event
+    n = c.n
+    lmax, nnbmax = 600, 550
+    lib = gpunb.load()
+    stream = gpunb.gpunb_stream()
+    d = {k: DeviceArray.from_numpy(a) for k, a in
+         (("x", c.x), ("v", c.v), ("dtr", stepr))}
+    outs = {}
+    for tag, fac in (("old", 0.9), ("new", 1.0)):
+        dh2 = DeviceArray.from_numpy(h2 * fac * fac)
+        o = {k: DeviceArray(sh, dt) for k, sh, dt in
+             (("acc", (n, 3), np.float64), ("jrk", (n, 3), np.float64),
+              ("pot", (n,), np.float64), ("nnb", (n,), np.int32),
+              ("off", (n,), np.int32), ("lst", (n * nnbmax,), np.int32),
+              ("tot", (1,), np.int32))}
+        rc = lib.gpunb_regf_dev(n, dh2.ptr, d["dtr"].ptr, d["x"].ptr,
+                                d["v"].ptr, o["acc"].ptr, o["jrk"].ptr,
+                                o["pot"].ptr, nnbmax, 0, o["nnb"].ptr, 0,
+                                o["lst"].ptr, o["off"].ptr, o["tot"].ptr,
+                                n * nnbmax)
+        if rc != 0 or lib.gpunb_sync() != 0:
+            gpunb._check()
+        outs[tag] = o
+    old = outs["old"]
+    LIST = regcor.compact_to_rows(old["nnb"].numpy(), old["off"].numpy(),
+                                  old["lst"].numpy(), lmax)
+    rng = np.random.default_rng(11)
+    state = {k: rng.normal(0.0, 1.0, (n, 3)) for k in
+             ("fi", "fidot", "fr", "frdot")}
+    for k in ("xn", "xndot", "d0", "d0r", "d1r", "d2r", "d3r"):
+        state[k] = np.zeros((n, 3))
+    state.update(x0=c.x, x0dot=c.v, t0r=1.0 - stepr, stepr=stepr.copy(),
+                 step=stepr / 2.0 ** rng.integers(0, 5, n), t0=np.ones(n),
+                 rs=np.sqrt(h2) * 0.9, body=c.m, list=LIST)
+    state["timenw"] = state["t0"] + state["step"]
+    par_kw = dict(time=1.0, rscale=1.0, rdens=(0.0, 0.0, 0.0), rc2=0.05 ** 2,
+                  rsfac=2.0, znbmin=10.0, smax=1.0, dtmin=1e-9, etar=0.02,
+                  smin=0.0, bodym=float(c.m.mean()), tidal4=0.0, nnbopt=56,
+                  nnbmax=nnbmax, n=n, ntot=n, ifirst=1, lmax=lmax, nb_flag=1,
+                  m_flag=0)
+    par = regcor.make_par(**par_kw)
+    new = outs["new"]
+    pristine = regcor.DeviceState(state)
+    work = regcor.DeviceState(state)
+    freg0 = DeviceArray((n, 3), np.float64).copy_from(new["acc"])
+    fdr0 = DeviceArray((n, 3), np.float64).copy_from(new["jrk"])
+    idx = DeviceArray.from_numpy(np.arange(1, n + 1, dtype=np.int32))
+    cnt = DeviceArray((4,), np.int32)
+    times = []
+    for r in range(reps + 1):
+        work.copy_from(pristine)
+        new["acc"].copy_from(freg0)
+        new["jrk"].copy_from(fdr0)
+        cnt.zero()
+        e0, e1 = Event(), Event()
+        e0.record(stream)
+        regcor.correct(par, work, n, idx.ptr, d["x"].ptr, d["v"].ptr,
+                       new["acc"].ptr, new["jrk"].ptr, new["nnb"].ptr,
+                       new["off"].ptr, new["lst"].ptr, n * nnbmax, cnt.ptr)
+        e1.record(stream)
+        if lib.gpunb_sync() != 0:
+            gpunb._check()
+        got = cnt.numpy().tolist()
+        if r == 0:
+            first = got
+        elif got != first:
+            raise RuntimeError(f"regcor counters differ between calls on the "
+                               f"same input: {got} vs {first}")
+        if r:                      # the first call warms up
+            times.append(e0.elapsed_ms(e1))
+    counts = dict(zip(gpunb.REGCOR_COUNTERS, first))
+    nnb_new = new["nnb"].numpy().astype(np.int64)
+    nnb0 = LIST[:, 0].astype(np.int64)
+    flux = counts["nbflux"]
+    # Algorithmic bytes of one call: per body the block inputs (idx, x, v,
+    # FREG, FDR, count, offset: 108 B), the state read (X0, X0DOT, FI,
+    # FIDOT, FR, FRDOT, T0R, STEPR, STEP, TIMENW, RS: 184 B) and written (XN,
+    # XNDOT, FI, FIDOT, FR, FRDOT, D0, D0R..D3R, T0R, RS, STEPR, FREG, FDR,
+    # STEP, TIMENW: 368 B) plus the step staging (32 B); per list entry 4 B
+    # (new list read, NLIST staged, old LIST read, new LIST written); per
+    # gained or lost neighbour its 56-B record, its STEP and 4 B of JJLIST.
+    nbytes = (n * (108 + 184 + 368 + 32) +
+              4 * int(nnb_new.sum() + (nnb_new + 1).sum() + (nnb0 + 1).sum()
+                      + nnb_new.sum()) + 68 * flux)
+    ms = float(np.median(times))
+    out = {"workload": f"King W0=6 N={n}: every body corrected after a "
+                       "regular step whose lists grew from 0.9 RS to RS",
+           "bodies": n, "counters": counts, "kernel_ms_median": ms,
+           "kernel_ms": times, "bodies_per_s": n / (ms * 1e-3),
+           "algorithmic_bytes": nbytes,
+           "achieved_GBps": nbytes / (ms * 1e-3) / 1e9,
+           "peak_GBps": 8000.0}
+    if cpu:
+        from oracle import regcor as orc
+        freg_h, fdr_h = freg0.numpy(), fdr0.numpy()
+        rows = regcor.compact_to_rows(nnb_new, new["off"].numpy(),
+                                      new["lst"].numpy(), lmax)
+        st = {k: (np.array(v, dtype=np.int32) if k == "list" else
+                  np.array(v, dtype=np.float64)) for k, v in state.items()}
+        st["x"], st["xdot"] = c.x, c.v
+        t0 = time.perf_counter()
+        orc.regcor(orc.make_par(**par_kw), st, np.arange(1, n + 1), c.x, c.v,
+                   freg_h, fdr_h, rows)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": n / dt, "unit": "bodies/s",
+                               "cores": 1, "kind": "port",
+                               "sample": f"oracle/regcor.c, all {n} bodies "
+                                         f"once, {dt:.3f} s"}
+    return out
+
+
 def cpu_baseline(c, h2, stepr, min_s=10.0):
     """Reference AVX regular force (oracle/_ref) on the same j set: repeated
     sweeps over all i in 1024-i calls until at least min_s seconds."""
@@ -107,6 +228,8 @@ def main():
                     help="skip the CPU baseline leg")
     ap.add_argument("--prec", type=int, default=0,
                     help="pair arithmetic: 32, 64 or 65 (0 = library default)")
+    ap.add_argument("--no-regcor", action="store_true",
+                    help="skip the device regular-corrector leg")
     ap.add_argument("--no-hermite", action="store_true",
                     help="skip the Hermite/Ahmad-Cohen integration leg")
     ap.add_argument("--no-ensemble", action="store_true",
@@ -234,6 +357,13 @@ def main():
             call(q)
         abi_rate = max(abi_rate, nchunk * ci * n / (time.perf_counter() - t2))
     gpunb._check()
+
+    # Regular corrector on the device for every body of the block (while the
+    # engine still holds the sent bodies as its j records).
+    regcor_leg = None
+    if not args.no_regcor:
+        log("bench: regular corrector (gpunb_regcor_dev), all bodies")
+        regcor_leg = regcor_bench(c, h2, stepr, cpu=not args.no_cpu)
     gpunb.gpunb_close()
 
     cpu = None
@@ -288,6 +418,13 @@ def main():
         t_gra = ens.time_step(eb, nnb_e, "graph", reps=5)
         group, rounds, tiles, t_tuned = ens.autotune(eb, nnb_e, reps=3, log=log)
         t_final = ens.time_step(eb, nnb_e, "graph", reps=10)
+        # The plain FP64 pair kernel, one launch per cluster at the default
+        # decomposition: the baseline of the mixed-precision batched form.
+        gpunb.gpunb_ens_set_bucket_tiles([])
+        gpunb.gpunb_set_tuning(0, 0.0)
+        gpunb.gpunb_set_precision(64)
+        t_seq64 = ens.time_step(eb, nnb_e, "sequential", reps=3)
+        gpunb.gpunb_set_precision(0)
         gpunb.gpunb_ens_close()
         gpunb.gpunb_set_tuning(0, 0.0)
         ensemble = {
@@ -299,6 +436,8 @@ def main():
             "batched_pairs_per_s": pairs / t_bat,
             "graph_pairs_per_s": pairs / t_gra,
             "tuned_graph_pairs_per_s": pairs / t_final,
+            "fp64_sequential_pairs_per_s": pairs / t_seq64,
+            "mixed_batched_over_fp64_sequential": t_seq64 / t_final,
             "tuned": {"group": group, "rounds": rounds,
                       "bucket_tiles": {f"2^{b}": t for b, t in enumerate(tiles) if t}},
         }
@@ -363,6 +502,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "abi_ni1024_pairs_per_s": abi_rate,
+        "regcor": regcor_leg,
         "hermite_1tnb": hermite,
         "ensemble": ensemble,
         "learned_rs": learned,

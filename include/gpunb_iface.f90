@@ -5,8 +5,22 @@
 ! between "CALL GPUNB_REGF(...)" (implicit, as util_gpu.F:60 does) and the
 ! checked interface below without any change in the bytes passed.
 module gpunb_iface
-  use iso_c_binding, only: c_int, c_double
+  use iso_c_binding, only: c_int, c_double, c_long_long, c_ptr
   implicit none
+
+  ! gpunb_regcor_par / gpunb_regcor_state of include/gpunb.h: the common6.h
+  ! scalars REGCOR_GPU reads, and device addresses of the body arrays.
+  type, bind(C) :: gpunb_regcor_par
+    real(c_double) :: time, rscale, rdens(3), rc2, rsfac, znbmin, smax, &
+                      dtmin, etar, smin, bodym, tidal4
+    integer(c_int) :: nnbopt, nnbmax, n, ntot, ifirst, lmax, nb_flag, m_flag
+  end type
+  type, bind(C) :: gpunb_regcor_state
+    type(c_ptr) :: x0, x0dot, xn, xndot, fi, fidot, fr, frdot, d0, d0r, &
+                   d1r, d2r, d3r, t0r, stepr, step, t0, timenw, rs, body, h, &
+                   list
+  end type
+
   interface
     ! src/Main/gpunb.gpu.cu:822-824
     subroutine gpunb_devinit(irank) bind(C, name="gpunb_devinit_")
@@ -107,5 +121,19 @@ module gpunb_iface
       real(c_double), intent(out) :: acc(3,*), jrk(3,*)
       integer(c_int), intent(out) :: nnbid(*)
     end subroutine
+    ! Device regular corrector for a block (replaces the REGCOR_GPU loop of
+    ! util_gpu.F:152-180); d_* are device addresses.
+    function gpunb_regcor_dev(par, st, ni, d_idx, d_xi, d_vi, d_freg, d_fdr, &
+                              d_nnb, d_off, d_list, list_len, d_counters) &
+        result(rc) bind(C, name="gpunb_regcor_dev")
+      import :: c_int, c_long_long, c_ptr, gpunb_regcor_par, gpunb_regcor_state
+      type(gpunb_regcor_par), intent(in) :: par
+      type(gpunb_regcor_state), intent(in) :: st
+      integer(c_int), value :: ni
+      type(c_ptr), value :: d_idx, d_xi, d_vi, d_freg, d_fdr, d_nnb, d_off, &
+                            d_list, d_counters
+      integer(c_long_long), value :: list_len
+      integer(c_int) :: rc
+    end function
   end interface
 end module gpunb_iface

@@ -270,9 +270,9 @@ int pick_group() {
     if (g && atoi(g) == 2) return 2;
     if (g && atoi(g) == 4) return 4;
     // Two pairs per group: 90 VGPRs, 5 resident waves per SIMD (four pairs:
-    // 132 VGPRs, 3 waves), which hides the latency of the dependent FP32 /
-    // FP64 chains better; measured 4.04 ms against 5.29 ms per 4.3e9-pair
-    // launch on MI355X.
+    // more than 128 VGPRs, 3 waves), which hides the latency of the
+    // dependent FP32 / FP64 chains better; measured 3.26 ms against 4.07 ms
+    // per 4.3e9-pair launch on MI355X.
     return 2;
 }
 
@@ -281,7 +281,8 @@ double pick_rounds() {
     const char *r = getenv("GPUNB_WAVE_ROUNDS");
     if (r && atof(r) > 0) return atof(r);
     // Four times the resident-wave capacity per launch: smaller tasks even
-    // out the tail (N = 131072: 4.02 ms at 2, 3.70 ms at 4, 3.70 ms at 6).
+    // out the tail (N = 131072: 3.51 ms at 2, 3.26 ms at 4); whole steps at
+    // 4, 6, 8 and 12 rounds are within 2% for blocks of 128 to 131072.
     return 4.0;
 }
 
@@ -1133,6 +1134,13 @@ GPUNB_EXPORT int gpunb_regf_dev(int ni, const double *d_h2, const double *d_dtr,
                      m_flag, d_nnb, d_nnid, d_list, d_off, d_total, list_cap);
     });
 }
+
+// Layouts shared with ctypes (gpunb.RegcorPar / RegcorState) and the
+// Fortran bind(C) types of gpunb_iface.f90.
+static_assert(sizeof(gpunb_regcor_par) == 14 * 8 + 8 * 4,
+              "gpunb_regcor_par layout");
+static_assert(sizeof(gpunb_regcor_state) == 22 * sizeof(void *),
+              "gpunb_regcor_state layout");
 
 GPUNB_EXPORT int gpunb_regcor_dev(const gpunb_regcor_par *par,
                                   const gpunb_regcor_state *st, int ni,

@@ -77,6 +77,16 @@ class DeviceArray:
         check(rt().hipMemset(self.ptr, 0, self.nbytes), "hipMemset")
         return self
 
+    def copy_from(self, other: "DeviceArray"):
+        """Device-to-device copy of another array of the same size, complete
+        on return (a device-to-device hipMemcpy may return before the copy
+        is done, and other streams are not ordered after it)."""
+        assert other.nbytes == self.nbytes
+        check(rt().hipMemcpy(self.ptr, other.ptr, self.nbytes, D2D),
+              "hipMemcpy")
+        check(rt().hipDeviceSynchronize(), "hipDeviceSynchronize")
+        return self
+
     def numpy(self):
         out = np.empty(self.shape, self.dtype)
         check(rt().hipMemcpy(out.ctypes.data, self.ptr, self.nbytes, D2H),

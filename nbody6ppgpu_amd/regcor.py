@@ -69,6 +69,38 @@ class DeviceState:
         out = {k: a.numpy() for k, a in self.arr.items() if a is not None}
         return out
 
+    def copy_from(self, other: "DeviceState") -> None:
+        """Device-to-device copy of another state of the same shapes."""
+        for k, a in self.arr.items():
+            if a is not None:
+                a.copy_from(other.arr[k])
+
+
+def compact_to_rows(nnb, off, lst, lmax: int, first: int = 1) -> np.ndarray:
+    """gpunb_regf_dev's compact lists (0-based j, body included) as LIST /
+    NLIST rows (util_gpu.F:103-110): count, then the 1-based members
+    j + first without the body itself.  Shape (ni, lmax) int32."""
+    nnb = np.asarray(nnb, dtype=np.int64)
+    off = np.asarray(off, dtype=np.int64)
+    ni = len(nnb)
+    if np.any(nnb < 1) or np.any(off < 0):
+        raise ValueError("every list must hold the body itself, no overflow")
+    row = np.repeat(np.arange(ni), nnb)
+    k = np.arange(int(nnb.sum())) - np.repeat(np.cumsum(nnb) - nnb, nnb)
+    mem = np.asarray(lst)[np.repeat(off, nnb) + k] + first
+    keep = mem != row + first
+    if int((~keep).sum()) != ni:
+        raise ValueError("each list must contain its body exactly once")
+    kself = np.empty(ni, dtype=np.int64)
+    kself[row[~keep]] = k[~keep]
+    col = 1 + k - (k > kself[row])
+    if int(nnb.max(initial=1)) + 1 > lmax:
+        raise ValueError("lmax too small for these lists")
+    rows = np.zeros((ni, lmax), dtype=np.int32)
+    rows[:, 0] = nnb - 1
+    rows[row[keep], col[keep]] = mem[keep]
+    return rows
+
 
 def correct(par: RegcorPar, dev: DeviceState, ni: int, d_idx, d_xi, d_vi,
             d_freg, d_fdr, d_nnb, d_off, d_list, list_len: int,
