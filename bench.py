@@ -182,6 +182,72 @@ def regcor_bench(c, h2, stepr, reps=5, cpu=True):
     return out
 
 
+def energy_bench(reps=5, cpu=True):
+    """Energy sum on the device (gpunb_energy_dev): a Plummer system of
+    N = 131072 (126976 singles and 2048 KS pairs), neighbour lists within
+    0.08 and pair perturber lists within 0.1, c.m. resolution radius 0.03.
+    HIP events around each call, once with the device potential (the O(N^2)
+    gpupot kernels) and once with the corrections and sums alone; the CPU
+    restatement of ENERGY + PHICOR (serial, as the reference's loop runs)
+    on the same system as the baseline of the corrections."""
+    from nbody6ppgpu_amd import energy, gpunb, ics
+    from nbody6ppgpu_amd.hiprt import DeviceArray, Event
+    s = ics.ks_system(nsingle=126976, npairs=2048, seed=21, rnb=0.08,
+                      rpert=0.1)
+    cut = 0.03 ** 2
+    par = energy.make_par(s["n"], s["ntot"], s["lmax"], cut)
+    d = {k: DeviceArray.from_numpy(np.ascontiguousarray(s[k])) for k in
+         ("body", "x", "xdot", "list")}
+    phi = DeviceArray((s["ntot"],), np.float64).zero()
+    out = DeviceArray((2,), np.float64)
+    stream = gpunb.gpunb_stream()
+    lib = gpunb.load()
+    res = {}
+    for mode, run_pot in (("with_potential", True), ("corrections_only", False)):
+        times = []
+        for r in range(reps + 1):
+            e0, e1 = Event(), Event()
+            e0.record(stream)
+            energy.energy_dev(par, d["body"].ptr, d["x"].ptr, d["xdot"].ptr,
+                              d["list"].ptr, run_pot, phi.ptr, out.ptr)
+            e1.record(stream)
+            if lib.gpunb_sync() != 0:
+                gpunb._check()
+            if r:
+                times.append(e0.elapsed_ms(e1))
+        res[mode] = float(np.median(times))
+        if run_pot:
+            pot_zkin = out.numpy().tolist()
+    nb = s["ntot"] - s["ifirst"] + 1
+    leg = {"workload": "Plummer N=131072: 126976 singles + 2048 KS pairs, "
+                       "lists within 0.08 (pairs: 0.1), resolution 0.03",
+           "bodies": nb, "pot_zkin": pot_zkin,
+           "ms_with_potential": res["with_potential"],
+           "ms_corrections_only": res["corrections_only"],
+           "bodies_per_s_corrections": nb / (res["corrections_only"] * 1e-3)}
+    if cpu:
+        from oracle import energy as orc
+        # The same potentials through gpupot_ (same kernels), corrected and
+        # summed by the serial restatement: timed, and its POT must agree.
+        ifirst = s["ifirst"]
+        phi_in = np.zeros(s["ntot"])
+        phi_in[ifirst - 1:] = gpunb.gpupot(1, nb, s["body"][ifirst - 1:],
+                                           s["x"][ifirst - 1:])
+        t0 = time.perf_counter()
+        _, pot, zkin = orc.energy(s["n"], s["ntot"], s["lmax"], cut, s["body"],
+                                  s["x"], s["xdot"], s["list"], phi_in)
+        dt = time.perf_counter() - t0
+        if abs(pot - pot_zkin[0]) > 1e-12 * abs(pot):
+            raise RuntimeError(f"energy sum: device POT {pot_zkin[0]!r} "
+                               f"against {pot!r}")
+        leg["cpu_baseline"] = {"value": nb / dt, "unit": "bodies/s",
+                               "cores": 1, "kind": "port",
+                               "sample": f"oracle/energy.c corrections and "
+                                         f"sums of all {nb} bodies, "
+                                         f"{dt * 1e3:.1f} ms"}
+    return leg
+
+
 def cpu_baseline(c, h2, stepr, min_s=10.0):
     """Reference AVX regular force (oracle/_ref) on the same j set: repeated
     sweeps over all i in 1024-i calls until at least min_s seconds."""
@@ -230,6 +296,8 @@ def main():
                     help="pair arithmetic: 32, 64 or 65 (0 = library default)")
     ap.add_argument("--no-regcor", action="store_true",
                     help="skip the device regular-corrector leg")
+    ap.add_argument("--no-energy", action="store_true",
+                    help="skip the device energy-sum leg")
     ap.add_argument("--no-hermite", action="store_true",
                     help="skip the Hermite/Ahmad-Cohen integration leg")
     ap.add_argument("--no-ensemble", action="store_true",
@@ -365,6 +433,12 @@ def main():
         log("bench: regular corrector (gpunb_regcor_dev), all bodies")
         regcor_leg = regcor_bench(c, h2, stepr, cpu=not args.no_cpu)
     gpunb.gpunb_close()
+
+    # Potential-energy sum of the energy check with KS pairs on the device.
+    energy_leg = None
+    if not args.no_energy:
+        log("bench: energy sum (gpunb_energy_dev), N=131072 with KS pairs")
+        energy_leg = energy_bench(cpu=not args.no_cpu)
 
     cpu = None
     if not args.no_cpu:
@@ -503,6 +577,7 @@ def main():
         "cpu_baseline": cpu,
         "abi_ni1024_pairs_per_s": abi_rate,
         "regcor": regcor_leg,
+        "energy": energy_leg,
         "hermite_1tnb": hermite,
         "ensemble": ensemble,
         "learned_rs": learned,

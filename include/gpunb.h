@@ -372,6 +372,36 @@ int gpunb_regcor_dev(const gpunb_regcor_par *par,
                      double *d_fdr, const int *d_nnb, const int *d_off,
                      const int *d_list, long long list_len, int *d_counters);
 
+/* ------------------------------------------------------------------------ */
+/* Potential energy of the energy check on the device: the GPU branch of     */
+/* ENERGY (energy.F:37-79: GPUPOT over IFIRST..NTOT, the c.m. potential      */
+/* copied to the KS components, PHICOR's differential corrections, phicor.f: */
+/* 1-112) and the kinetic energy (energy.F:141-146).                         */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    int n;               /* N: singles and KS components are 1..N */
+    int ntot;            /* NTOT = N + NPAIRS (c.m. bodies N+1..NTOT) */
+    int ifirst;          /* IFIRST = 2*NPAIRS + 1 */
+    int lmax;            /* LMAX: row length of LIST */
+    double cmsep2_rmin2; /* CMSEP2*RMIN2: radius^2 of the c.m. resolution */
+} gpunb_energy_par;
+
+/* Device arrays indexed by body I - 1: d_body[NTOT], d_x / d_xdot
+ * [NTOT][3] (components at their resolved positions), d_list LIST(LMAX,NTOT)
+ * (singles: neighbour lists with c.m. members last; KS pair IPAIR: its
+ * perturber list in row 2*IPAIR-1), d_phidbl[NTOT] PHIDBL.  run_pot != 0:
+ * PHIDBL(IFIRST..NTOT) is computed first by the gpupot_ kernels; 0: the
+ * caller's values are used.  On return (after gpunb_sync) PHIDBL holds the
+ * corrected potentials of singles and components, d_out[0] = POT and
+ * d_out[1] = ZKIN (both halved as energy.F does).  Asynchronous on the
+ * library stream; invalid list rows are reported by gpunb_sync().
+ * Returns 0, or -1 with gpunb_last_error() set. */
+int gpunb_energy_dev(const gpunb_energy_par *par, const double *d_body,
+                     const double *d_x, const double *d_xdot,
+                     const int *d_list, int run_pot, double *d_phidbl,
+                     double *d_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,162 @@
+// energy.hip -- the potential-energy sum of NBODY6++GPU's energy check on the
+// device: the GPU branch of ENERGY (src/Main/energy.F:37-79) with the
+// differential corrections of PHICOR (src/Main/phicor.f:1-112) and the
+// kinetic energy (energy.F:141-146).
+//
+//   1. PHIDBL(IFIRST..NTOT) from the O(N^2) potential kernels of gpupot_
+//      (optional: a caller may pass its own PHIDBL);
+//   2. per body I = IFIRST..NTOT (one thread each):
+//        single (I <= N): the c.m. neighbours at the end of LIST(.,I) inside
+//          CMSEP2*RMIN2 are replaced by their two KS components;
+//        c.m. (I > N): every perturber of the pair (LIST(.,I1)) is resolved
+//          against each component; PHIDBL(I1), PHIDBL(I1+1) are set to the
+//          c.m. value plus the component's correction;
+//   3. POT = 1/2 sum BODY * PHIDBL over singles and KS components and
+//      ZKIN = 1/2 sum BODY |XDOT|^2 over IFIRST..NTOT, each summed by one
+//      workgroup in a fixed order (deterministic run to run; the reference's
+//      serial sum differs only in association).
+//
+// The corrections use the reference's operations in its order (the library
+// is built without FMA contraction, division and sqrt are correctly
+// rounded), so PHIDBL matches a serial restatement bit for bit.
+
+#include "gpunb_internal.h"
+
+namespace gpunb {
+
+namespace {
+
+__device__ inline double d2(const double *x, int a, int b) {
+    // RIJ2 = 0; RIJ2 = RIJ2 + (X(K,a) - X(K,b))**2, K = 1..3
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double d = x[3 * (size_t)(a - 1) + k] - x[3 * (size_t)(b - 1) + k];
+        s = s + d * d;
+    }
+    return s;
+}
+
+__global__ void k_energy_corr(EnergyLaunch L) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int I = L.ifirst + t;
+    if (I > L.ntot) return;
+    const int N = L.n;
+    const double *X = L.x, *B = L.body;
+    double *PHI = L.phidbl;
+    double dphi1 = 0.0, dphi2 = 0.0;
+    bool ok = true;
+    auto valid = [&](int J) { return J >= 1 && J <= L.ntot; };
+    if (I <= N) {
+        const int *row = L.list + (size_t)(I - 1) * L.lmax;
+        const int nnb = row[0];
+        ok = nnb >= 0 && nnb + 1 <= L.lmax;
+        // Backwards until the last c.m. body (phicor.f:14-39).
+        for (int l = ok ? nnb + 1 : 1; l >= 2; --l) {
+            const int J = row[l - 1];
+            if (!valid(J)) {
+                ok = false;
+                break;
+            }
+            if (J <= N) break;
+            const double rij2 = d2(X, I, J);
+            if (rij2 < L.cmsep2_rmin2) {
+                const int j1 = 2 * (J - N) - 1, j2 = j1 + 1;
+                const double rj1 = d2(X, I, j1);
+                const double rj2 = d2(X, I, j2);
+                dphi1 = dphi1 - B[J - 1] / sqrt(rij2) + B[j1 - 1] / sqrt(rj1) +
+                        B[j2 - 1] / sqrt(rj2);
+            }
+        }
+        double c = 0.0;
+        if (ok) {
+            const double phi = PHI[I - 1] + dphi1;
+            PHI[I - 1] = phi;
+            c = B[I - 1] * phi;
+        }
+        L.cpot[t] = c;
+    } else {
+        // KS pair c.m.: perturbers of the pair (phicor.f:40-108).
+        const int i1 = 2 * (I - N) - 1, i2 = i1 + 1;
+        const int *row = L.list + (size_t)(i1 - 1) * L.lmax;
+        const int np = row[0];
+        ok = np >= 0 && np + 1 <= L.lmax;
+        for (int l = 2; ok && l <= np + 1; ++l) {
+            const int J = row[l - 1];
+            if (!valid(J)) {
+                ok = false;
+                break;
+            }
+            const double rij2 = d2(X, I, J);
+            if (J <= N) {
+                const double ri1 = d2(X, i1, J);
+                const double ri2 = d2(X, i2, J);
+                dphi1 = dphi1 - B[J - 1] / sqrt(rij2) + B[J - 1] / sqrt(ri1);
+                dphi2 = dphi2 - B[J - 1] / sqrt(rij2) + B[J - 1] / sqrt(ri2);
+            } else {
+                dphi1 = dphi1 - B[J - 1] / sqrt(rij2);
+                dphi2 = dphi2 - B[J - 1] / sqrt(rij2);
+                const int j1 = 2 * (J - N) - 1;
+                for (int jj = j1; jj <= j1 + 1; ++jj) {
+                    const double rj1 = d2(X, i1, jj);
+                    const double rj2 = d2(X, i2, jj);
+                    dphi1 = dphi1 + B[jj - 1] / sqrt(rj1);
+                    dphi2 = dphi2 + B[jj - 1] / sqrt(rj2);
+                }
+            }
+        }
+        double c = 0.0;
+        if (ok) {
+            const double pcm = PHI[I - 1];
+            const double p1 = pcm + dphi1, p2 = pcm + dphi2;
+            PHI[i1 - 1] = p1;
+            PHI[i2 - 1] = p2;
+            c = B[i1 - 1] * p1 + B[i2 - 1] * p2;
+        }
+        L.cpot[t] = c;
+    }
+    if (!ok) atomicOr(L.err, ERR_ENERGY);
+    const double *v = L.xdot + 3 * (size_t)(I - 1);
+    L.ckin[t] = B[I - 1] * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+}
+
+// One workgroup of 1024: thread t sums items t, t + 1024, ... in order,
+// then a fixed-shape tree in LDS.  out[0] = POT / 2, out[1] = ZKIN / 2.
+__global__ __launch_bounds__(1024) void k_energy_sum(const double *cpot,
+                                                     const double *ckin,
+                                                     int n, double *out) {
+    __shared__ double sp[1024], sk[1024];
+    const int t = threadIdx.x;
+    double p = 0.0, k = 0.0;
+    for (int i = t; i < n; i += 1024) {
+        p += cpot[i];
+        k += ckin[i];
+    }
+    sp[t] = p;
+    sk[t] = k;
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+        if (t < s) {
+            sp[t] += sp[t + s];
+            sk[t] += sk[t + s];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        out[0] = 0.5 * sp[0];
+        out[1] = 0.5 * sk[0];
+    }
+}
+
+}  // namespace
+
+void launch_energy(hipStream_t st, const EnergyLaunch &L) {
+    const int nb = L.ntot - L.ifirst + 1;
+    if (nb <= 0) return;
+    hipLaunchKernelGGL(k_energy_corr, dim3((unsigned)((nb + 255) / 256)),
+                       dim3(256), 0, st, L);
+    hipLaunchKernelGGL(k_energy_sum, dim3(1), dim3(1024), 0, st, L.cpot,
+                       L.ckin, nb, L.out);
+}
+
+}  // namespace gpunb

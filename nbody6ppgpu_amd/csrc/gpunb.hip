@@ -144,6 +144,9 @@ struct State {
     DevBuf<int> rc_nlist, rc_jj, rc_mark;
     DevBuf<double> rc_step, rc_tnw;
 
+    // Energy-sum scratch (gpunb_energy_dev).
+    DevBuf<double> en_cpot, en_ckin;
+
     double time_send = 0, time_grav = 0, time_nb = 0, time_out = 0;
     long long numInter = 0;
     int icall = 0, ini = 0, isend = 0;
@@ -257,6 +260,9 @@ void err_check() {
     if (e & ERR_NAN_J) m += "NaN in j-particle data; ";
     if (e & ERR_NAN_I) m += "NaN in i-particle data; ";
     if (e & ERR_LIST_CAP) m += "neighbour list buffer too small; ";
+    if (e & ERR_ENERGY)
+        m += "energy sum: a LIST row with a count out of range or a member "
+             "outside 1..NTOT; ";
     if (e & ERR_REGCOR)
         m += "regular corrector: body index out of range or listed twice, or "
              "a list too long, unsorted or outside the j set (those bodies "
@@ -516,6 +522,8 @@ void do_close() {
     S.rc_mark.release();
     S.rc_step.release();
     S.rc_tnw.release();
+    S.en_cpot.release();
+    S.en_ckin.release();
     S.h_list.release();
     S.nbmax = 0;
     S.nj = 0;
@@ -1134,6 +1142,64 @@ GPUNB_EXPORT int gpunb_regf_dev(int ni, const double *d_h2, const double *d_dtr,
                      m_flag, d_nnb, d_nnid, d_list, d_off, d_total, list_cap);
     });
 }
+
+GPUNB_EXPORT int gpunb_energy_dev(const gpunb_energy_par *par,
+                                  const double *d_body, const double *d_x,
+                                  const double *d_xdot, const int *d_list,
+                                  int run_pot, double *d_phidbl,
+                                  double *d_out) {
+    return guarded("gpunb_energy_dev", [&] {
+        REQUIRE(par, "gpunb_energy_dev: par must not be NULL");
+        const gpunb_energy_par &P = *par;
+        REQUIRE(P.n >= 0 && P.ntot >= P.n && P.ntot >= 1,
+                "gpunb_energy_dev: need 0 <= n <= ntot, ntot >= 1");
+        REQUIRE(P.ifirst == 2 * (P.ntot - P.n) + 1 && P.ifirst <= P.n + 1,
+                "gpunb_energy_dev: need ifirst = 2*npairs + 1 with npairs = "
+                "ntot - n");
+        REQUIRE(P.lmax >= 1, "gpunb_energy_dev: lmax must be >= 1");
+        REQUIRE(d_body && d_x && d_xdot && d_list && d_phidbl && d_out,
+                "gpunb_energy_dev: a device array is NULL");
+        devinit(0);
+        set_device();
+        const int nb = P.ntot - P.ifirst + 1;
+        if (run_pot && nb > 0) {
+            const int ntile = cdiv(nb, kTile);
+            const int n_pad = ntile * kTile;
+            S.pot_rec.ensure((size_t)n_pad + kTile);
+            launch_pack_p(S.st, d_body + (P.ifirst - 1),
+                          d_x + 3 * (size_t)(P.ifirst - 1), nb, n_pad + kTile,
+                          S.pot_rec.p, S.err.p);
+            if (!S.pot_resident) S.pot_resident = pot_resident_waves();
+            const int target = std::max(S.pot_resident, 1024) * 2;
+            const int n_iw = cdiv(nb, 64);
+            int ns = std::max(1, (int)std::lround((double)target / n_iw));
+            ns = std::min(ns, std::max(ntile, 1));
+            S.d_ppart.ensure((size_t)round_up(nb, 64) * ns);
+            launch_pot(S.st, S.pot_rec.p, nb, ntile, 0, nb, ns, S.d_ppart.p,
+                       d_phidbl + (P.ifirst - 1));
+        }
+        S.en_cpot.ensure((size_t)std::max(nb, 1));
+        S.en_ckin.ensure((size_t)std::max(nb, 1));
+        EnergyLaunch L;
+        L.n = P.n;
+        L.ntot = P.ntot;
+        L.ifirst = P.ifirst;
+        L.lmax = P.lmax;
+        L.cmsep2_rmin2 = P.cmsep2_rmin2;
+        L.body = d_body;
+        L.x = d_x;
+        L.xdot = d_xdot;
+        L.list = d_list;
+        L.phidbl = d_phidbl;
+        L.cpot = S.en_cpot.p;
+        L.ckin = S.en_ckin.p;
+        L.out = d_out;
+        L.err = S.err.p;
+        launch_energy(S.st, L);
+    });
+}
+
+static_assert(sizeof(gpunb_energy_par) == 4 * 4 + 8, "gpunb_energy_par layout");
 
 // Layouts shared with ctypes (gpunb.RegcorPar / RegcorState) and the
 // Fortran bind(C) types of gpunb_iface.f90.

@@ -47,6 +47,7 @@ enum : unsigned {
     ERR_NAN_I = 2u,      // NaN in i data (regf)
     ERR_LIST_CAP = 4u,   // compact list buffer too small (device entry point)
     ERR_REGCOR = 8u,     // regular corrector: invalid body index or list
+    ERR_ENERGY = 16u,    // energy sum: invalid list row
 };
 
 // Far-away parking place of the padding j records (mass 0).
@@ -150,6 +151,19 @@ struct RegcorLaunch {
     unsigned *err;
 };
 void launch_regcor(hipStream_t st, const RegcorLaunch &L);
+
+// One potential-energy sum (energy.hip).
+struct EnergyLaunch {
+    int n, ntot, ifirst, lmax;
+    double cmsep2_rmin2;
+    const double *body, *x, *xdot;
+    const int *list;
+    double *phidbl;
+    double *cpot, *ckin;          // scratch [ntot - ifirst + 1]
+    double *out;                  // [2]: POT, ZKIN
+    unsigned *err;
+};
+void launch_energy(hipStream_t st, const EnergyLaunch &L);
 
 // Occupancy-derived resident-wave capacity of the regular-force kernel.
 int regf_resident_waves(int m_flag, int group, int prec);

@@ -84,6 +84,13 @@ class RegcorState(ctypes.Structure):
                 [("h", ctypes.c_void_p), ("list", ctypes.c_void_p)])
 
 
+class EnergyPar(ctypes.Structure):
+    """gpunb_energy_par: body counts and the c.m. resolution radius."""
+    _fields_ = [("n", ctypes.c_int), ("ntot", ctypes.c_int),
+                ("ifirst", ctypes.c_int), ("lmax", ctypes.c_int),
+                ("cmsep2_rmin2", ctypes.c_double)]
+
+
 def _declare(lib: ctypes.CDLL) -> None:
     v = None
     sig = {
@@ -111,6 +118,10 @@ def _declare(lib: ctypes.CDLL) -> None:
                               ctypes.POINTER(RegcorState), ctypes.c_int] +
                              [ctypes.c_void_p] * 8 +
                              [ctypes.c_longlong, ctypes.c_void_p]),
+        "gpunb_energy_dev": (ctypes.c_int,
+                             [ctypes.POINTER(EnergyPar)] +
+                             [ctypes.c_void_p] * 4 + [ctypes.c_int] +
+                             [ctypes.c_void_p] * 2),
         "gpunb_sync": (ctypes.c_int, []),
         "gpunb_stream": (ctypes.c_void_p, []),
         "gpunb_version": (ctypes.c_char_p, []),

@@ -286,3 +286,66 @@ def energy(m, x, v, pot_positive):
     ke = 0.5 * np.sum(m * np.sum(v * v, axis=1))
     pe = -0.5 * np.sum(m * pot_positive)
     return ke + pe, ke, pe
+
+
+def ks_system(nsingle=600, npairs=40, seed=3, rnb=0.3, rpert=0.4, lmax=None,
+              complete=False, sep=2e-3):
+    """A Plummer system of nsingle singles and npairs KS pairs in NBODY6++'s
+    body order: components 1..2 NPAIRS (pair p: 2p-1, 2p, at separation
+    ~sep about their c.m.), singles 2 NPAIRS+1..N, c.m. bodies N+1..NTOT.
+    LIST rows as common6.h holds them: a single's neighbours within rnb
+    (singles and c.m. bodies, ascending), pair p's perturbers within rpert of
+    its c.m. in row 2p-1 (complete=True: every body of IFIRST..NTOT); other
+    rows empty.  Returns dict(n, ntot, ifirst, npairs, lmax, body, x, xdot,
+    list)."""
+    from scipy.spatial import cKDTree
+    rng = np.random.default_rng(seed)
+    nsys = nsingle + npairs
+    c = plummer(nsys, seed=seed)
+    N = 2 * npairs + nsingle
+    ntot = N + npairs
+    ifirst = 2 * npairs + 1
+    body = np.zeros(ntot)
+    x = np.zeros((ntot, 3))
+    v = np.zeros((ntot, 3))
+    # systems 0..npairs-1 are pairs, the rest singles
+    for p in range(npairs):
+        mc, xc, vc = c.m[p], c.x[p], c.v[p]
+        q = rng.uniform(0.2, 0.8)
+        m1, m2 = q * mc, (1 - q) * mc
+        d = rng.normal(size=3)
+        d *= sep * rng.uniform(0.5, 1.5) / np.linalg.norm(d)
+        u = rng.normal(size=3) * 0.5
+        i1, i2, icm = 2 * p, 2 * p + 1, N + p
+        body[i1], body[i2], body[icm] = m1, m2, mc
+        x[i1], x[i2], x[icm] = xc + (m2 / mc) * d, xc - (m1 / mc) * d, xc
+        v[i1], v[i2], v[icm] = vc + (m2 / mc) * u, vc - (m1 / mc) * u, vc
+    for s in range(nsingle):
+        i = ifirst - 1 + s
+        body[i], x[i], v[i] = c.m[npairs + s], c.x[npairs + s], c.v[npairs + s]
+    # bodies that take part in the potential: IFIRST..NTOT (0-based ifirst-1..)
+    active = np.arange(ifirst - 1, ntot)
+    tree = cKDTree(x[active])
+    rows = [[] for _ in range(ntot)]
+    singles = np.arange(ifirst - 1, N)
+    cms = np.arange(N, ntot)
+    if complete:
+        near_s = [list(range(len(active)))] * nsingle
+        near_p = [list(range(len(active)))] * npairs
+    else:
+        near_s = tree.query_ball_point(x[singles], rnb) if nsingle else []
+        near_p = tree.query_ball_point(x[cms], rpert) if npairs else []
+    for s, i in enumerate(singles):
+        nb = active[np.asarray(near_s[s], dtype=np.int64)]
+        rows[i] = sorted(int(j) + 1 for j in nb if j != i)
+    for p, icm in enumerate(cms):
+        nb = active[np.asarray(near_p[p], dtype=np.int64)]
+        rows[2 * p] = sorted(int(j) + 1 for j in nb if j != icm)
+    if lmax is None:
+        lmax = max(len(r) for r in rows) + 8
+    lst = np.zeros((ntot, lmax), np.int32)
+    for i, r in enumerate(rows):
+        lst[i, 0] = len(r)
+        lst[i, 1:1 + len(r)] = r
+    return dict(n=N, ntot=ntot, ifirst=ifirst, npairs=npairs, lmax=lmax,
+                body=body, x=x, xdot=v, list=lst)
