@@ -12,9 +12,9 @@
 //          against each component; PHIDBL(I1), PHIDBL(I1+1) are set to the
 //          c.m. value plus the component's correction;
 //   3. POT = 1/2 sum BODY * PHIDBL over singles and KS components and
-//      ZKIN = 1/2 sum BODY |XDOT|^2 over IFIRST..NTOT, each summed by one
-//      workgroup in a fixed order (deterministic run to run; the reference's
-//      serial sum differs only in association).
+//      ZKIN = 1/2 sum BODY |XDOT|^2 over IFIRST..NTOT, summed in a fixed
+//      two-level order (deterministic run to run; the reference's serial
+//      sum differs only in association).
 //
 // The corrections use the reference's operations in its order (the library
 // is built without FMA contraction, division and sqrt are correctly
@@ -120,28 +120,65 @@ __global__ void k_energy_corr(EnergyLaunch L) {
     L.ckin[t] = B[I - 1] * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
 }
 
-// One workgroup of 1024: thread t sums items t, t + 1024, ... in order,
-// then a fixed-shape tree in LDS.  out[0] = POT / 2, out[1] = ZKIN / 2.
-__global__ __launch_bounds__(1024) void k_energy_sum(const double *cpot,
-                                                     const double *ckin,
-                                                     int n, double *out) {
-    __shared__ double sp[1024], sk[1024];
-    const int t = threadIdx.x;
-    double p = 0.0, k = 0.0;
-    for (int i = t; i < n; i += 1024) {
-        p += cpot[i];
-        k += ckin[i];
-    }
-    sp[t] = p;
-    sk[t] = k;
+// Deterministic two-level sum.  Level 1: block b of 256 threads sums items
+// [2048 b, 2048 (b + 1)), eight per thread in order, then a fixed tree in
+// LDS.  Level 2: one block sums the level-1 partials in the same way and
+// writes out[0] = POT / 2, out[1] = ZKIN / 2.
+constexpr int kSumItems = 2048;
+
+__device__ inline void block_tree(double *sp, double *sk, int t) {
     __syncthreads();
-    for (int s = 512; s > 0; s >>= 1) {
+    for (int s = 128; s > 0; s >>= 1) {
         if (t < s) {
             sp[t] += sp[t + s];
             sk[t] += sk[t + s];
         }
         __syncthreads();
     }
+}
+
+__global__ __launch_bounds__(256) void k_energy_sum1(const double *cpot,
+                                                     const double *ckin,
+                                                     int n, double *ppart,
+                                                     double *kpart) {
+    __shared__ double sp[256], sk[256];
+    const int t = threadIdx.x;
+    const int base = blockIdx.x * kSumItems;
+    double vp[8], vk[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        const int i = base + u * 256 + t;
+        vp[u] = i < n ? cpot[i] : 0.0;
+        vk[u] = i < n ? ckin[i] : 0.0;
+    }
+    double p = 0.0, k = 0.0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        p += vp[u];
+        k += vk[u];
+    }
+    sp[t] = p;
+    sk[t] = k;
+    block_tree(sp, sk, t);
+    if (t == 0) {
+        ppart[blockIdx.x] = sp[0];
+        kpart[blockIdx.x] = sk[0];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_energy_sum2(const double *ppart,
+                                                     const double *kpart,
+                                                     int nb, double *out) {
+    __shared__ double sp[256], sk[256];
+    const int t = threadIdx.x;
+    double p = 0.0, k = 0.0;
+    for (int i = t; i < nb; i += 256) {
+        p += ppart[i];
+        k += kpart[i];
+    }
+    sp[t] = p;
+    sk[t] = k;
+    block_tree(sp, sk, t);
     if (t == 0) {
         out[0] = 0.5 * sp[0];
         out[1] = 0.5 * sk[0];
@@ -155,8 +192,11 @@ void launch_energy(hipStream_t st, const EnergyLaunch &L) {
     if (nb <= 0) return;
     hipLaunchKernelGGL(k_energy_corr, dim3((unsigned)((nb + 255) / 256)),
                        dim3(256), 0, st, L);
-    hipLaunchKernelGGL(k_energy_sum, dim3(1), dim3(1024), 0, st, L.cpot,
-                       L.ckin, nb, L.out);
+    const int nsum = (nb + kSumItems - 1) / kSumItems;
+    hipLaunchKernelGGL(k_energy_sum1, dim3((unsigned)nsum), dim3(256), 0, st,
+                       L.cpot, L.ckin, nb, L.ppart, L.kpart);
+    hipLaunchKernelGGL(k_energy_sum2, dim3(1), dim3(256), 0, st, L.ppart,
+                       L.kpart, nsum, L.out);
 }
 
 }  // namespace gpunb

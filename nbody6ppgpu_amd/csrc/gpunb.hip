@@ -145,7 +145,7 @@ struct State {
     DevBuf<double> rc_step, rc_tnw;
 
     // Energy-sum scratch (gpunb_energy_dev).
-    DevBuf<double> en_cpot, en_ckin;
+    DevBuf<double> en_cpot, en_ckin, en_part;
 
     double time_send = 0, time_grav = 0, time_nb = 0, time_out = 0;
     long long numInter = 0;
@@ -524,6 +524,7 @@ void do_close() {
     S.rc_tnw.release();
     S.en_cpot.release();
     S.en_ckin.release();
+    S.en_part.release();
     S.h_list.release();
     S.nbmax = 0;
     S.nj = 0;
@@ -1180,6 +1181,7 @@ GPUNB_EXPORT int gpunb_energy_dev(const gpunb_energy_par *par,
         }
         S.en_cpot.ensure((size_t)std::max(nb, 1));
         S.en_ckin.ensure((size_t)std::max(nb, 1));
+        S.en_part.ensure(2 * (size_t)std::max((nb + 2047) / 2048, 1));
         EnergyLaunch L;
         L.n = P.n;
         L.ntot = P.ntot;
@@ -1193,6 +1195,8 @@ GPUNB_EXPORT int gpunb_energy_dev(const gpunb_energy_par *par,
         L.phidbl = d_phidbl;
         L.cpot = S.en_cpot.p;
         L.ckin = S.en_ckin.p;
+        L.ppart = S.en_part.p;
+        L.kpart = S.en_part.p + std::max((nb + 2047) / 2048, 1);
         L.out = d_out;
         L.err = S.err.p;
         launch_energy(S.st, L);

@@ -160,6 +160,7 @@ struct EnergyLaunch {
     const int *list;
     double *phidbl;
     double *cpot, *ckin;          // scratch [ntot - ifirst + 1]
+    double *ppart, *kpart;        // scratch [ceil((ntot - ifirst + 1) / 2048)]
     double *out;                  // [2]: POT, ZKIN
     unsigned *err;
 };

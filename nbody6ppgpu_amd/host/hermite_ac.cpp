@@ -38,7 +38,8 @@
 //
 // Usage:  hermite_ac <in.bin> <t_end> [options]   (see parse_args)
 // in.bin: int32 n, then FP64 m[n], x[n][3], v[n][3].
-// Output: one JSON object per energy check on stdout.
+// Output: one JSON object per energy check on stdout; with --conf3 DIR also
+// a conf.3 snapshot per check (the reference's binary format, output.F).
 #include "gpunb.h"
 
 #include <omp.h>
@@ -79,6 +80,9 @@ struct Opts {
     double nnbmin = 0.0;       // 0: ZNBMIN = max(0.01 NNBMAX, 1) (input.F:122)
     std::string mlp_path;      // weights file for --rs mlp
     std::string log_path;      // training pairs out (--log-train)
+    std::string conf3_dir;     // conf.3 snapshots at every energy check
+    std::string dump_path;     // checkpoint written at every energy check
+    std::string restart_path;  // checkpoint to continue from
     long long log_max = 2000000;
     unsigned log_seed = 12345;
     double progress = 0.0;     // > 0: a progress line on stderr every so many s
@@ -231,6 +235,9 @@ struct Run {
     double rdens[3] = {0.0, 0.0, 0.0};
     double rscale = 1.0, rc2 = 1.0, rsfac = 10.0, znbmin = 1.0, nnbmin = 1.0;
     double tcr = 2.0 * std::sqrt(2.0), vc = std::sqrt(0.5), mmean = 1.0;
+    // Five-neighbour density and number density per body (core.f:90-98),
+    // from the last update_centre; potentials of the last energy check.
+    std::vector<double> rho_b, xn_b, phidbl;
     // Radius statistics.
     long long learned_steps = 0, fallbacks = 0, nnb_dev_n = 0;
     double nnb_dev_sum = 0.0;
@@ -475,7 +482,7 @@ struct Run {
     // Density centre, core radius, half-mass radius and RSFAC from the
     // current state (every body at t0 == t) and lists.
     void update_centre() {
-        std::vector<double> rho(n, 0.0);
+        std::vector<double> rho(n, 0.0), xn(n, 0.0);
 #pragma omp parallel for schedule(dynamic, 256)
         for (int q = 0; q < n; ++q) {
             const Body &b = B[q];
@@ -495,8 +502,13 @@ struct Run {
             double ms = 0.0;
             for (int u = 0; u < 5; ++u) ms += dm[u].second;
             const double r6 = std::sqrt(dm[5].first);
-            if (r6 > 0.0) rho[q] = ms / (r6 * r6 * r6);
+            if (r6 > 0.0) {
+                xn[q] = 1.0 / (r6 * r6 * r6);
+                rho[q] = ms * xn[q];
+            }
         }
+        rho_b = rho;
+        xn_b = xn;
         double sw = 0.0, sx[3] = {0, 0, 0};
         for (int q = 0; q < n; ++q) {
             sw += rho[q];
@@ -739,9 +751,267 @@ struct Run {
         t_pot += wtime() - w0;
         double pe = 0.0;
         for (int q = 0; q < n; ++q) pe -= 0.5 * m[q] * ph[q];
+        phidbl = ph;
         *ekin = ke;
         *epot = pe;
         return ke + pe;
+    }
+
+    // Checkpoint of the whole integration state at an energy check (all
+    // bodies synchronised): the options that shape the run, the cluster
+    // quantities, the counters, E0 and every body's state and list.  A run
+    // restarted from it continues bit for bit as the uninterrupted run does
+    // (the device state is rebuilt from the bodies; wall-clock timers are
+    // not part of it).  Layout: "NB6AMDCK", int32 version, int32 n, then
+    // native doubles / int64 / int32 as written below.
+    static constexpr int kCkptVersion = 1;
+    void dump(const std::string &path, double e0) const {
+        const std::string tmp = path + ".tmp";
+        FILE *f = fopen(tmp.c_str(), "wb");
+        if (!f) {
+            perror(tmp.c_str());
+            exit(1);
+        }
+        bool ok = true;
+        auto put = [&](const void *p, size_t bytes) {
+            ok = ok && fwrite(p, 1, bytes, f) == bytes;
+        };
+        put("NB6AMDCK", 8);
+        const int32_t hdr[2] = {kCkptVersion, n};
+        put(hdr, sizeof hdr);
+        const double od[4] = {o.etai, o.etar, o.dtadj, o.dtmin};
+        const int32_t oi[8] = {o.nnbopt, o.lmax, o.nnbmax, o.rs_rule,
+                               o.irr_dev, o.prec, o.use_predict ? 1 : 0,
+                               o.irr_min};
+        put(od, sizeof od);
+        put(oi, sizeof oi);
+        const double sc[14] = {t, e0, rdens[0], rdens[1], rdens[2], rscale,
+                               rc2, rsfac, znbmin, nnbmin, tcr, vc, mmean,
+                               nnb_dev_sum};
+        put(sc, sizeof sc);
+        const long long cnt[11] = {reg_pairs, irr_pairs, reg_calls, irr_steps,
+                                   reg_steps, blocks, overflows, learned_steps,
+                                   fallbacks, nnb_dev_n, log_seen};
+        put(cnt, sizeof cnt);
+        for (const Body &b : B) {
+            put(&b.m, sizeof b.m);
+            put(b.x0, sizeof b.x0);
+            put(b.v0, sizeof b.v0);
+            put(b.a, sizeof b.a);
+            put(b.j, sizeof b.j);
+            put(b.fi, sizeof b.fi);
+            put(b.fid, sizeof b.fid);
+            put(b.fi2, sizeof b.fi2);
+            put(b.fi3, sizeof b.fi3);
+            put(b.fr, sizeof b.fr);
+            put(b.frd, sizeof b.frd);
+            put(b.fr2, sizeof b.fr2);
+            put(b.fr3, sizeof b.fr3);
+            const double tt[5] = {b.t0, b.dt, b.t0r, b.dtr, b.rs};
+            put(tt, sizeof tt);
+            put(b.feat, sizeof b.feat);
+            const int32_t fl[3] = {b.has_feat ? 1 : 0, b.learned ? 1 : 0,
+                                   (int32_t)b.list.size()};
+            put(fl, sizeof fl);
+            put(b.list.data(), b.list.size() * sizeof(int));
+        }
+        ok = (fclose(f) == 0) && ok;
+        if (!ok || rename(tmp.c_str(), path.c_str()) != 0) {
+            fprintf(stderr, "checkpoint: write to %s failed\n", path.c_str());
+            exit(1);
+        }
+    }
+
+    // Restores a checkpoint written by dump() for the same n and options;
+    // returns E0.  The device state (predictor records, irregular-force
+    // records and lists) and the block schedule are rebuilt from the bodies.
+    double restore(const std::string &path) {
+        FILE *f = fopen(path.c_str(), "rb");
+        if (!f) {
+            perror(path.c_str());
+            exit(1);
+        }
+        auto get = [&](void *p, size_t bytes) {
+            if (fread(p, 1, bytes, f) != bytes) {
+                fprintf(stderr, "checkpoint %s: truncated\n", path.c_str());
+                exit(1);
+            }
+        };
+        auto fail = [&](const char *what) {
+            fprintf(stderr, "checkpoint %s: %s\n", path.c_str(), what);
+            exit(1);
+        };
+        char magic[8];
+        get(magic, 8);
+        if (memcmp(magic, "NB6AMDCK", 8) != 0) fail("not a checkpoint");
+        int32_t hdr[2];
+        get(hdr, sizeof hdr);
+        if (hdr[0] != kCkptVersion) fail("unknown version");
+        if (hdr[1] != n) fail("written for another N");
+        double od[4];
+        int32_t oi[8];
+        get(od, sizeof od);
+        get(oi, sizeof oi);
+        const double odn[4] = {o.etai, o.etar, o.dtadj, o.dtmin};
+        const int32_t oin[8] = {o.nnbopt, o.lmax, o.nnbmax, o.rs_rule,
+                                o.irr_dev, o.prec, o.use_predict ? 1 : 0,
+                                o.irr_min};
+        if (memcmp(od, odn, sizeof od) != 0 || memcmp(oi, oin, sizeof oi) != 0)
+            fail("written with other integration options");
+        double sc[14];
+        get(sc, sizeof sc);
+        t = sc[0];
+        const double e0 = sc[1];
+        rdens[0] = sc[2];
+        rdens[1] = sc[3];
+        rdens[2] = sc[4];
+        rscale = sc[5];
+        rc2 = sc[6];
+        rsfac = sc[7];
+        znbmin = sc[8];
+        nnbmin = sc[9];
+        tcr = sc[10];
+        vc = sc[11];
+        mmean = sc[12];
+        nnb_dev_sum = sc[13];
+        long long cnt[11];
+        get(cnt, sizeof cnt);
+        reg_pairs = cnt[0];
+        irr_pairs = cnt[1];
+        reg_calls = cnt[2];
+        irr_steps = cnt[3];
+        reg_steps = cnt[4];
+        blocks = cnt[5];
+        overflows = cnt[6];
+        learned_steps = cnt[7];
+        fallbacks = cnt[8];
+        nnb_dev_n = cnt[9];
+        log_seen = cnt[10];
+        B.assign(n, Body{});
+        for (Body &b : B) {
+            get(&b.m, sizeof b.m);
+            get(b.x0, sizeof b.x0);
+            get(b.v0, sizeof b.v0);
+            get(b.a, sizeof b.a);
+            get(b.j, sizeof b.j);
+            get(b.fi, sizeof b.fi);
+            get(b.fid, sizeof b.fid);
+            get(b.fi2, sizeof b.fi2);
+            get(b.fi3, sizeof b.fi3);
+            get(b.fr, sizeof b.fr);
+            get(b.frd, sizeof b.frd);
+            get(b.fr2, sizeof b.fr2);
+            get(b.fr3, sizeof b.fr3);
+            double tt[5];
+            get(tt, sizeof tt);
+            b.t0 = tt[0];
+            b.dt = tt[1];
+            b.t0r = tt[2];
+            b.dtr = tt[3];
+            b.rs = tt[4];
+            get(b.feat, sizeof b.feat);
+            int32_t fl[3];
+            get(fl, sizeof fl);
+            b.has_feat = fl[0] != 0;
+            b.learned = fl[1] != 0;
+            if (fl[2] < 0 || fl[2] > o.lmax) fail("bad list length");
+            b.list.resize(fl[2]);
+            get(b.list.data(), b.list.size() * sizeof(int));
+            for (int jb : b.list)
+                if (jb < 0 || jb >= n) fail("bad list member");
+        }
+        fclose(f);
+        rpos.assign(n, -1);
+        rk.assign(n, -1);
+        due.clear();
+        for (int q = 0; q < n; ++q) due[B[q].t0 + B[q].dt].push_back(q);
+        std::vector<int> all(n);
+        for (int q = 0; q < n; ++q) all[q] = q;
+        jpset(all);
+        if (o.irr_dev) {
+            std::vector<std::vector<int>> L(n);
+            for (int q = 0; q < n; ++q) L[q] = B[q].list;
+            irr_lists(all, L);
+        }
+        return e0;
+    }
+
+    // conf.3 snapshot of the synchronised state at t (output.F:488-535):
+    // records (NTOT, MODEL, NRUN, NK) and (AS(1..20), BODYS, RHOS, XNS, XS,
+    // VS, PHI = -PHIDBL, NAME), REAL*4 and INTEGER*4, each framed by 4-byte
+    // byte counts as Fortran unformatted sequential files are; file name
+    // conf.3_ + string_left(t, dtadj).  N-body units: RBAR = ZMBAR = TSCALE
+    // = VSTAR = 1; RTIDE, TIDAL(4), CMAX and DMIN1 (no tidal field, no
+    // close-encounter bookkeeping here) are 0.
+    void write_conf3(double tt) const {
+        int ip = 0;
+        double dtmp = o.dtadj;
+        while (dtmp - std::trunc(dtmp) != 0.0 && ip < 40) {
+            ++ip;
+            dtmp *= 10.0;
+        }
+        char stamp[64];
+        snprintf(stamp, sizeof stamp, "%.*f", ip, tt);
+        std::string st(stamp);
+        if (st.size() > 20) st = std::string(20, '*');
+        const std::string path = o.conf3_dir + "/conf.3_" + st;
+        const long long rec2 = 4LL * (20 + 11LL * n);
+        if (rec2 >= (1LL << 31)) {
+            fprintf(stderr, "conf.3: NTOT too large for one record\n");
+            exit(1);
+        }
+        const double rc = std::sqrt(rc2);
+        int nc = 0;
+        double rhom = 0.0;
+        for (int q = 0; q < n; ++q) {
+            double r2 = 0.0;
+            for (int c = 0; c < 3; ++c) {
+                const double e = B[q].x0[c] - rdens[c];
+                r2 += e * e;
+            }
+            if (r2 < rc2) ++nc;
+            if (q < (int)rho_b.size()) rhom = std::max(rhom, rho_b[q]);
+        }
+        double rsmin = 1e30;
+        for (int q = 0; q < n; ++q) rsmin = std::min(rsmin, B[q].rs);
+        const float as[20] = {(float)tt, 0.0f, 1.0f, 1.0f, 0.0f, 0.0f,
+                              (float)rdens[0], (float)rdens[1], (float)rdens[2],
+                              (float)(tt / tcr), 1.0f, 1.0f, (float)rc,
+                              (float)nc, (float)vc, (float)rhom, 0.0f,
+                              (float)rscale, (float)rsmin, 0.0f};
+        std::vector<float> fl;
+        fl.reserve(20 + 10 * (size_t)n);
+        fl.insert(fl.end(), as, as + 20);
+        for (int q = 0; q < n; ++q) fl.push_back((float)B[q].m);
+        for (int q = 0; q < n; ++q)
+            fl.push_back(q < (int)rho_b.size() ? (float)rho_b[q] : 0.0f);
+        for (int q = 0; q < n; ++q)
+            fl.push_back(q < (int)xn_b.size() ? (float)xn_b[q] : 0.0f);
+        for (int q = 0; q < n; ++q)
+            for (int c = 0; c < 3; ++c) fl.push_back((float)B[q].x0[c]);
+        for (int q = 0; q < n; ++q)
+            for (int c = 0; c < 3; ++c) fl.push_back((float)B[q].v0[c]);
+        for (int q = 0; q < n; ++q)
+            fl.push_back(q < (int)phidbl.size() ? (float)(-phidbl[q]) : 0.0f);
+        std::vector<int32_t> name(n);
+        for (int q = 0; q < n; ++q) name[q] = q + 1;
+        FILE *f = fopen(path.c_str(), "wb");
+        if (!f) {
+            perror(path.c_str());
+            exit(1);
+        }
+        const int32_t r1[4] = {n, 1, 1, 20};
+        const int32_t l1 = 16, l2 = (int32_t)rec2;
+        bool ok = fwrite(&l1, 4, 1, f) == 1 && fwrite(r1, 4, 4, f) == 4 &&
+                  fwrite(&l1, 4, 1, f) == 1 && fwrite(&l2, 4, 1, f) == 1 &&
+                  fwrite(fl.data(), 4, fl.size(), f) == fl.size() &&
+                  fwrite(name.data(), 4, name.size(), f) == name.size() &&
+                  fwrite(&l2, 4, 1, f) == 1;
+        ok = (fclose(f) == 0) && ok;
+        if (!ok) {
+            fprintf(stderr, "conf.3: write to %s failed\n", path.c_str());
+            exit(1);
+        }
     }
 
     // One block step to the earliest due time.
@@ -1047,7 +1317,7 @@ void parse_args(int argc, char **argv, Opts &o, std::string &in) {
                 "[--rs0 x] [--rs ac|simple|mlp] [--mlp weights] "
                 "[--nnbmin x] [--log-train file] [--log-max n] "
                 "[--log-seed s] [--progress seconds] [--irr device|host] "
-                "[--irr-min k]\n",
+                "[--irr-min k] [--conf3 dir] [--dump file] [--restart file]\n",
                 argv[0]);
         exit(1);
     }
@@ -1078,6 +1348,9 @@ void parse_args(int argc, char **argv, Opts &o, std::string &in) {
         else if (s == "--mlp") o.mlp_path = nxt();
         else if (s == "--nnbmin") o.nnbmin = atof(nxt());
         else if (s == "--log-train") o.log_path = nxt();
+        else if (s == "--conf3") o.conf3_dir = nxt();
+        else if (s == "--dump") o.dump_path = nxt();
+        else if (s == "--restart") o.restart_path = nxt();
         else if (s == "--log-max") o.log_max = atoll(nxt());
         else if (s == "--log-seed") o.log_seed = (unsigned)atol(nxt());
         else if (s == "--progress") o.progress = atof(nxt());
@@ -1151,11 +1424,22 @@ int main(int argc, char **argv) {
         }
         R.load_mlp();
     }
+    if (!o.log_path.empty() && (!o.dump_path.empty() || !o.restart_path.empty())) {
+        fprintf(stderr, "--log-train cannot be combined with checkpoints\n");
+        return 1;
+    }
     double w0 = wtime();
-    R.init(m, x, v);
-    double ek, ep;
-    const double e0 = R.energy(&ek, &ep);
-    {
+    double ek, ep, e0;
+    if (!o.restart_path.empty()) {
+        // Continue from a checkpoint (written at an energy check).
+        e0 = R.restore(o.restart_path);
+        const double e = R.energy(&ek, &ep);
+        printf("{\"t\": %.6f, \"E\": %.15e, \"EK\": %.15e, \"EP\": %.15e, "
+               "\"dE_E\": %.6e, \"restart\": true}\n",
+               R.t, e, ek, ep, (e - e0) / std::fabs(e0));
+    } else {
+        R.init(m, x, v);
+        e0 = R.energy(&ek, &ep);
         // Crossing time and velocity scale of scale.F, then the cluster
         // quantities of the radius rule.
         double mt = 0.0;
@@ -1166,11 +1450,12 @@ int main(int argc, char **argv) {
         R.znbmin = std::max(0.01 * o.nnbmax, 1.0);
         R.nnbmin = o.nnbmin > 0.0 ? o.nnbmin : R.znbmin;
         R.update_centre();
+        if (!o.conf3_dir.empty()) R.write_conf3(0.0);
+        printf("{\"t\": 0.0, \"E\": %.15e, \"EK\": %.15e, \"EP\": %.15e, "
+               "\"dE_E\": 0.0}\n", e0, ek, ep);
     }
-    printf("{\"t\": 0.0, \"E\": %.15e, \"EK\": %.15e, \"EP\": %.15e, \"dE_E\": 0.0}\n",
-           e0, ek, ep);
     fflush(stdout);
-    double next = o.dtadj;
+    double next = R.t + o.dtadj;
     double last_progress = wtime();
     while (R.t < o.t_end) {
         R.block();
@@ -1206,6 +1491,8 @@ int main(int argc, char **argv) {
             fflush(stdout);
             next += o.dtadj;
             R.update_centre();
+            if (!o.conf3_dir.empty()) R.write_conf3(R.t);
+            if (!o.dump_path.empty()) R.dump(o.dump_path, e0);
         }
     }
     R.write_log();
