@@ -282,14 +282,17 @@ int pick_group() {
     return 2;
 }
 
-double pick_rounds() {
+double pick_rounds(int ni) {
     if (S.rounds > 0) return S.rounds;
     const char *r = getenv("GPUNB_WAVE_ROUNDS");
     if (r && atof(r) > 0) return atof(r);
-    // Four times the resident-wave capacity per launch: smaller tasks even
-    // out the tail (N = 131072: 3.51 ms at 2, 3.26 ms at 4); whole steps at
-    // 4, 6, 8 and 12 rounds are within 2% for blocks of 128 to 131072.
-    return 4.0;
+    // Resident-wave capacity multiples per launch, by block size: large
+    // blocks take 4 (smaller tasks even out the tail: N = 131072, 3.26 ms
+    // per launch at 4 against 3.51 at 2), small ones fewer, since every
+    // extra slice is one more FP64 partial per i to write and reduce
+    // (whole step, 1024 i: 0.187 ms at 1, 0.206 at 4; 128 i: 0.108 ms at
+    // 0.5, 0.121 at 4; 4096 i and up: 4 best).
+    return std::min(4.0, std::max(0.5, ni / 1024.0));
 }
 
 int pick_prec() {
@@ -318,7 +321,7 @@ Decomp decompose(int ni_c, int ntile, int nnbmax, int m_flag, int group,
                  int prec) {
     Decomp d;
     const int n_iw = cdiv(ni_c, 64);
-    const double target = resident_waves(m_flag, group, prec) * pick_rounds();
+    const double target = resident_waves(m_flag, group, prec) * pick_rounds(ni_c);
     int ns = (int)std::lround(target / n_iw);
     ns = std::max(ns, 1);
     ns = std::min(ns, std::max(ntile, 1));
@@ -901,7 +904,11 @@ void ens_send(int ncl, const int *nj, const double *m, const double (*x)[3],
 // `rounds` times the resident-wave capacity; a per-size-bucket table
 // (gpunb_ens_set_bucket_tiles, the autotuner's output) overrides it.
 int ens_slice_tiles(const int *ni, int group, int prec, int m_flag) {
-    const double target = resident_waves(m_flag, group, prec) * pick_rounds();
+    long long ni_all = 0;
+    for (int c = 0; c < E.ncl; ++c) ni_all += ni[c];
+    const double target =
+        resident_waves(m_flag, group, prec) *
+        pick_rounds((int)std::min<long long>(ni_all, 1 << 30));
     int lo = 1, hi = 1024;
     auto waves = [&](int L) {
         double w = 0;
