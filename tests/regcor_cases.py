@@ -44,7 +44,8 @@ def neighbours(x, rs):
 
 
 def build(n=2048, seed=5, lmax=600, nnbmax=500, nnbopt=50, dtr=1.0 / 64,
-          time=1.0, change=0.15, ncm=0, smin=0.0):
+          time=1.0, change=0.15, ncm=0, smin=0.0, hrange=(1.0, 50.0),
+          nb_flag=1, m_flag=0, tidal4=0.0):
     """A state for n bodies (the last ncm of them c.m. bodies I > N) whose
     lists change between the old radius rs and the new radius rs * (1 +/-
     change).  Returns (par_kw, state, block) with block = dict(idx, xi, vi,
@@ -92,14 +93,14 @@ def build(n=2048, seed=5, lmax=600, nnbmax=500, nnbopt=50, dtr=1.0 / 64,
     state["rs"] = rs_old.copy()
     state["body"] = m.copy()
     state["list"] = LIST
-    state["h"] = -rng.uniform(1.0, 50.0, max(ncm, 1))
+    state["h"] = -rng.uniform(hrange[0], hrange[1], max(ncm, 1))
     state["x"] = x
     state["xdot"] = v
     par_kw = dict(time=time, rscale=1.0, rdens=(0.01, -0.02, 0.005),
                   rc2=0.1 ** 2, rsfac=2.0, znbmin=0.2 * nnbopt, smax=1.0,
                   dtmin=1e-9, etar=0.02, smin=smin, bodym=float(m.mean()),
-                  tidal4=0.0, nnbopt=nnbopt, nnbmax=nnbmax, n=N, ntot=ntot,
-                  ifirst=1, lmax=lmax, nb_flag=1, m_flag=0)
+                  tidal4=tidal4, nnbopt=nnbopt, nnbmax=nnbmax, n=N, ntot=ntot,
+                  ifirst=1, lmax=lmax, nb_flag=nb_flag, m_flag=m_flag)
     # new lists in the compact regular-force form (0-based j, self included)
     idx = np.arange(1, ntot + 1, dtype=np.int32)
     nnb = np.zeros(ntot, np.int32)

@@ -71,8 +71,17 @@ def compare(got, want, tag):
     print(f"{tag}: {total} FP64 outputs bit-identical, counters {gc}")
 
 
-def test_block_matches_oracle():
-    par_kw, state, block = rc.build(n=4096, seed=31, ncm=64, smin=1e-4)
+@pytest.mark.parametrize("variant", ["nb_flag1", "nb_flag0_mflag1_tidal"])
+def test_block_matches_oracle(variant):
+    if variant == "nb_flag1":
+        par_kw, state, block = rc.build(n=4096, seed=31, ncm=64, smin=1e-4)
+    else:
+        # radii about the origin, mass-weighted lists (no short-list rule),
+        # a tidal FIDOT term, and c.m. bodies with small |H| so that their
+        # perturber-range branch (regcor_gpu.F:187-199) runs
+        par_kw, state, block = rc.build(n=4096, seed=31, ncm=256, smin=1e-4,
+                                        hrange=(1e-5, 1e-3), nb_flag=0,
+                                        m_flag=1, tidal4=0.3)
     rng = np.random.default_rng(3)
     # small steps for a few bodies so that the retention path runs
     state["step"][rng.choice(4096, 200, replace=False)] = 1e-5
@@ -98,7 +107,7 @@ def test_block_matches_oracle():
     got = device_run(par_kw, state, block)
     assert want[3]["nbvoid"] == 3 and want[3]["nbsmin"] > 0
     assert want[3]["nbflux"] > 0 and want[3]["niconv"] > 0
-    compare(got, want, "n=4096 with c.m. bodies")
+    compare(got, want, f"n=4096 with c.m. bodies, {variant}")
 
 
 def test_subset_block_and_repeat():
