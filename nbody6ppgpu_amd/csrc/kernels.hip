@@ -897,17 +897,30 @@ __global__ __launch_bounds__(64) void k_pot_pass1(const PRec *__restrict__ p,
     }
     float a = 0.f;
     double A = 0.0;
+    // Groups of four records; the next group is loaded while the current one
+    // is evaluated (the record array carries a spare tile past its end, so
+    // the last look-ahead stays in bounds).  Terms are added in j order.
+    constexpr int GP = 4;
+    PRec Jn[GP];
+#pragma unroll
+    for (int k = 0; k < GP; ++k) Jn[k] = p[tb * kTile + k];
     for (int t = tb; t < te; ++t) {
         const int jt = t * kTile;
-#pragma unroll 8
-        for (int jj = 0; jj < kTile; ++jj) {
-            const PRec J = p[jt + jj];
-            const float fx = (float)(J.x - x0);
-            const float fy = (float)(J.y - y0);
-            const float fz = (float)(J.z - z0);
-            const float r2 = fmaf(fz, fz, fmaf(fy, fy, fx * fx));
-            const float term = J.m32 * __builtin_amdgcn_rsqf(r2);
-            a += (r2 > 0.0f) ? term : 0.0f;
+        for (int jg = 0; jg < kTile; jg += GP) {
+            PRec J[GP];
+#pragma unroll
+            for (int k = 0; k < GP; ++k) J[k] = Jn[k];
+#pragma unroll
+            for (int k = 0; k < GP; ++k) Jn[k] = p[jt + jg + GP + k];
+#pragma unroll
+            for (int k = 0; k < GP; ++k) {
+                const float fx = (float)(J[k].x - x0);
+                const float fy = (float)(J[k].y - y0);
+                const float fz = (float)(J[k].z - z0);
+                const float r2 = fmaf(fz, fz, fmaf(fy, fy, fx * fx));
+                const float term = J[k].m32 * __builtin_amdgcn_rsqf(r2);
+                a += (r2 > 0.0f) ? term : 0.0f;
+            }
         }
         A += (double)a;
         a = 0.f;
