@@ -187,6 +187,24 @@ __global__ __launch_bounds__(256) void k_predict(
     const double *__restrict__ m, JRec *__restrict__ jp,
     JStats *__restrict__ jst) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    // The block's slices of the four (3,N) arrays are staged through LDS
+    // with unit-stride loads (a thread reading its own three components
+    // would touch each cache line in thirds).
+    __shared__ double sx[768], sv[768], sf[768], sfd[768];
+    {
+        const size_t base = 3 * (size_t)blockIdx.x * blockDim.x;
+        const size_t lim = 3 * (size_t)nj;
+        for (int e = threadIdx.x; e < 3 * (int)blockDim.x; e += blockDim.x) {
+            const size_t q = base + (size_t)e;
+            if (q < lim) {
+                sx[e] = x0[q];
+                sv[e] = v0[q];
+                sf[e] = f[q];
+                sfd[e] = fd[q];
+            }
+        }
+    }
+    __syncthreads();
     double vmag = 0.0, mpos = INFINITY;
     if (j < nj) {
         // xbpredall.f:18-26, evaluated exactly as written (no FMA).
@@ -196,9 +214,9 @@ __global__ __launch_bounds__(256) void k_predict(
         double xo[3], vo[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            const size_t q = 3 * (size_t)j + k;
-            xo[k] = ((fd[q] * s + f[q]) * s + v0[q]) * s + x0[q];
-            vo[k] = (fd[q] * s1 + f[q]) * s2 + v0[q];
+            const int e = 3 * (int)threadIdx.x + k;
+            xo[k] = ((sfd[e] * s + sf[e]) * s + sv[e]) * s + sx[e];
+            vo[k] = (sfd[e] * s1 + sf[e]) * s2 + sv[e];
         }
         JRec r;
         r.x = xo[0];
