@@ -143,8 +143,8 @@ struct RegcorLaunch {
     long long list_len;           // readable entries of lst
     const JRec *jp;               // predicted j records
     int nj;
-    int *nlist;                   // scratch [ni][lmax]: NLIST
-    int *jj;                      // scratch [ni][2 lmax]: JJLIST
+    int *nlist;                   // scratch [lmax][ni]: NLIST
+    int *jj;                      // scratch [2 lmax][ni]: JJLIST
     double *step_new, *tnw_new;   // scratch [ni]
     int *mark;                    // [ntot], zero between calls
     int *cnt;                     // [4] counters

@@ -33,6 +33,8 @@ __device__ inline double sqd(double a) { return a * a; }
 constexpr double kOne3 = 1.0 / 3.0;    // ONE3 (zero.f:113)
 constexpr double kOne12 = 1.0 / 12.0;  // ONE12 (zero.f:116)
 constexpr int kMaxHalve = 2200;        // > 2098 halvings: DBL_MAX -> 0
+constexpr int kChunk = 8;              // list entries loaded per chunk
+constexpr int kWin = 4;                // look-ahead of the list merge
 
 // Pair force / derivative of neighbour record R on (xi, xidot), added
 // (sign > 0) to or subtracted from (f, fd) as regcor_gpu.F:349-357 and
@@ -93,11 +95,15 @@ __global__ void k_regcor(RegcorLaunch L) {
     }
     const int lmax = P.lmax;
     int *LIST = S.list + (size_t)p * lmax;
-    int *nl = L.nlist + (size_t)q * lmax;
-    int *jj = L.jj + (size_t)q * 2 * lmax;
-#define NL(k) nl[(k) - 1]
+    // NLIST / JJLIST scratch interleaved by body (entry k of body q at
+    // (k - 1) ni + q): the lanes of a wavefront, which walk their lists
+    // nearly in step, then share cache lines.
+    int *nl = L.nlist + q;
+    int *jj = L.jj + q;
+    const size_t ni_s = (size_t)L.ni;
+#define NL(k) nl[(size_t)((k) - 1) * ni_s]
 #define LI(k) LIST[(k) - 1]
-#define JJ(k) jj[(k) - 1]
+#define JJ(k) jj[(size_t)((k) - 1) * ni_s]
     auto rec = [&](int J) -> const JRec & { return L.jp[J - P.ifirst]; };
     auto in_j = [&](int J) {
         return J >= P.ifirst && J - P.ifirst < L.nj && J <= P.ntot;
@@ -114,20 +120,47 @@ __global__ void k_regcor(RegcorLaunch L) {
               (long long)og + cg <= L.list_len && nnb0 >= 0 &&
               nnb0 + 1 <= lmax && (I <= P.n || S.h != nullptr);
     int nnb = 0;
+    // Both walks load kChunk entries at a time, so that one memory latency
+    // is paid per chunk rather than per entry.
     if (ok) {
-        for (int k = 0; k < cg; ++k) {
-            const int J = L.lst[og + k] + P.ifirst;
-            if (J == I) continue;
-            if (!in_j(J) || (nnb > 0 && J <= NL(nnb + 1)) || nnb + 3 > lmax) {
-                ok = false;
-                break;
+        int prev = 0;
+        for (int k0 = 0; ok && k0 < cg; k0 += kChunk) {
+            int v[kChunk];
+#pragma unroll
+            for (int u = 0; u < kChunk; ++u)
+                v[u] = k0 + u < cg ? L.lst[og + k0 + u] : 0;
+#pragma unroll
+            for (int u = 0; u < kChunk; ++u) {
+                if (k0 + u >= cg) break;
+                const int J = v[u] + P.ifirst;
+                if (J == I) continue;
+                if (!in_j(J) || (nnb > 0 && J <= prev) || nnb + 3 > lmax) {
+                    ok = false;
+                    break;
+                }
+                NL(nnb + 2) = J;
+                prev = J;
+                ++nnb;
             }
-            NL(nnb + 2) = J;
-            ++nnb;
         }
         NL(1) = nnb;
-        for (int l = 2; ok && l <= nnb0 + 1; ++l)
-            if (!in_j(LI(l)) || (l > 2 && LI(l) <= LI(l - 1))) ok = false;
+        prev = 0;
+        for (int l0 = 2; ok && l0 <= nnb0 + 1; l0 += kChunk) {
+            int v[kChunk];
+#pragma unroll
+            for (int u = 0; u < kChunk; ++u)
+                v[u] = l0 + u <= nnb0 + 1 ? LI(l0 + u) : 0;
+#pragma unroll
+            for (int u = 0; u < kChunk; ++u) {
+                const int l = l0 + u;
+                if (l > nnb0 + 1) break;
+                if (!in_j(v[u]) || (l > 2 && v[u] <= prev)) {
+                    ok = false;
+                    break;
+                }
+                prev = v[u];
+            }
+        }
     }
     if (!ok) {
         atomicOr(L.err, ERR_REGCOR);
@@ -266,29 +299,56 @@ __global__ void k_regcor(RegcorLaunch L) {
         int l = 2, lg = 2;
         NL(nnb + 2) = P.ntot + 1;
         NL(1) = LI(nnb0 + 1);
+        // wa[d] = LI(l + d), wb[d] = NL(lg + d): each step advances l and lg
+        // by 0 or 1, so the entries kWin ahead are loaded kWin steps before
+        // they are compared.  The loop writes LIST only at LI(l) (the
+        // sentinel, with l = NNB0 + 1 from then on) and never writes NLIST.
+        int wa[kWin], wb[kWin];
+#pragma unroll
+        for (int d = 0; d < kWin; ++d) {
+            wa[d] = l + d <= nnb0 + 1 ? LI(l + d) : 0;
+            wb[d] = lg + d <= nnb + 2 ? NL(lg + d) : 0;
+        }
         for (;;) {
-            if (LI(l) != NL(lg)) {
-                if (LI(l) >= NL(lg)) {
+            const int a = wa[0], b = wb[0];
+            int l2 = l, lg2 = lg;
+            if (a != b) {
+                if (a >= b) {
                     ++nbgain;
-                    JJ(nnb0 + nbgain) = NL(lg);
-                    --l;
+                    JJ(nnb0 + nbgain) = b;
+                    --l2;
                 } else {
                     ++nbloss;
-                    const int J = LI(l);
-                    JJ(nbloss) = J;
-                    if (S.step[J - 1] < P.smin) jmin = J;
-                    --lg;
+                    JJ(nbloss) = a;
+                    if (S.step[a - 1] < P.smin) jmin = a;
+                    --lg2;
                 }
             }
-            if (l <= nnb0) {
-                ++l;
-                ++lg;
-            } else if (lg <= nnb) {
-                ++lg;
-                LI(l) = P.ntot + 1;
+            bool sent = false;
+            if (l2 <= nnb0) {
+                ++l2;
+                ++lg2;
+            } else if (lg2 <= nnb) {
+                ++lg2;
+                LI(l2) = P.ntot + 1;       // l2 == l here
+                sent = true;
             } else {
                 break;
             }
+            if (l2 != l) {
+#pragma unroll
+                for (int d = 0; d + 1 < kWin; ++d) wa[d] = wa[d + 1];
+                wa[kWin - 1] = l + kWin <= nnb0 + 1 ? LI(l + kWin) : 0;
+            } else if (sent) {
+                wa[0] = P.ntot + 1;
+            }
+            if (lg2 != lg) {
+#pragma unroll
+                for (int d = 0; d + 1 < kWin; ++d) wb[d] = wb[d + 1];
+                wb[kWin - 1] = lg + kWin <= nnb + 2 ? NL(lg + kWin) : 0;
+            }
+            l = l2;
+            lg = lg2;
         }
         if (jmin != 0) {
             // Keep a lost neighbour with a small step inside 2 RS
@@ -328,11 +388,24 @@ __global__ void k_regcor(RegcorLaunch L) {
         }
     }
 
-    // Irregular force change of the list change (regcor_gpu.F:400-438).
-    for (int l = 1; l <= nbloss; ++l)
-        pair_add(rec(JJ(l)), xi, xidot, false, dfirr, dfd);
-    for (int l = 1; l <= nbgain; ++l)
-        pair_add(rec(JJ(nnb0 + l)), xi, xidot, true, dfirr, dfd);
+    // Irregular force change of the list change (regcor_gpu.F:400-438),
+    // in list order; the next member's index and record are loaded while
+    // the current pair is evaluated.
+    auto pair_run = [&](int first, int count, bool add) {
+        if (count <= 0) return;
+        int jn = count > 1 ? JJ(first + 1) : 0;
+        JRec nx = rec(JJ(first));
+        for (int l = 0; l < count; ++l) {
+            const JRec cur = nx;
+            if (l + 1 < count) {
+                nx = rec(jn);
+                if (l + 2 < count) jn = JJ(first + l + 2);
+            }
+            pair_add(cur, xi, xidot, add, dfirr, dfd);
+        }
+    };
+    pair_run(1, nbloss, false);
+    pair_run(nnb0 + 1, nbgain, true);
 
     // Corrector (regcor_gpu.F:440-551).
     const double dtsq = dtr * dtr;
@@ -390,7 +463,15 @@ __global__ void k_regcor(RegcorLaunch L) {
     // New list (regcor_gpu.F:605-613).
     if (nbloss + nbgain > 0) {
         LI(1) = nnb;
-        for (int l = 2; l <= nnb + 1; ++l) LI(l) = NL(l);
+        for (int l0 = 2; l0 <= nnb + 1; l0 += kChunk) {
+            int v[kChunk];
+#pragma unroll
+            for (int u = 0; u < kChunk; ++u)
+                v[u] = l0 + u <= nnb + 1 ? NL(l0 + u) : 0;
+#pragma unroll
+            for (int u = 0; u < kChunk; ++u)
+                if (l0 + u <= nnb + 1) LI(l0 + u) = v[u];
+        }
         c_flux = nbloss + nbgain;
     }
 
