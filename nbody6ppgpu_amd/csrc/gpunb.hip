@@ -127,12 +127,11 @@ struct State {
     DevBuf<double> part;
     DevBuf<int> cnt;
     DevBuf<uint16_t> seg;
-    DevBuf<double> d_out;
-    HostBuf<double> h_out;
-    DevBuf<int> d_iout;
-    HostBuf<int> h_iout;
-    DevBuf<int> d_list;
-    HostBuf<int> h_list;
+    // gpunb_regf_ results in one buffer, so that one copy brings them back:
+    // acc, jrk, pot (7 ni doubles), then nnb, off, nnid, total (3 ni + 1
+    // ints), then the compact list from a 256-byte boundary.
+    DevBuf<char> d_res;
+    HostBuf<char> h_res;
     long long list_est = 0;
 
     DevBuf<PRec> pot_rec;
@@ -515,11 +514,8 @@ void do_close() {
     S.part.release();
     S.cnt.release();
     S.seg.release();
-    S.d_out.release();
-    S.h_out.release();
-    S.d_iout.release();
-    S.h_iout.release();
-    S.d_list.release();
+    S.d_res.release();
+    S.h_res.release();
     S.rc_nlist.release();
     S.rc_jj.release();
     S.rc_mark.release();
@@ -528,7 +524,6 @@ void do_close() {
     S.en_cpot.release();
     S.en_ckin.release();
     S.en_part.release();
-    S.h_list.release();
     S.nbmax = 0;
     S.nj = 0;
     S.njp = 0;
@@ -592,33 +587,29 @@ void do_regf(int ni, const double *h2, const double *dtr, const double (*xi)[3],
                           hipMemcpyHostToDevice, S.st));
 
     const long long list_cap = std::max<long long>(1, (long long)ni * nnbmax);
-    S.d_out.ensure(7 * (size_t)ni);
-    S.h_out.ensure(7 * (size_t)ni);
-    S.d_iout.ensure(3 * (size_t)ni + 1);
-    S.h_iout.ensure(3 * (size_t)ni + 1);
-    S.d_list.ensure((size_t)list_cap);
-    double *d_acc = S.d_out.p, *d_jrk = d_acc + 3 * (size_t)ni,
+    const size_t off_i = 7 * (size_t)ni * sizeof(double);
+    const size_t off_l =
+        (off_i + (3 * (size_t)ni + 1) * sizeof(int) + 255) & ~(size_t)255;
+    S.d_res.ensure(off_l + (size_t)list_cap * sizeof(int));
+    double *d_acc = (double *)S.d_res.p, *d_jrk = d_acc + 3 * (size_t)ni,
            *d_pot = d_acc + 6 * (size_t)ni;
-    int *d_nnb = S.d_iout.p, *d_off = d_nnb + ni, *d_nnid = d_off + ni,
-        *d_total = d_nnid + ni;
+    int *d_nnb = (int *)(S.d_res.p + off_i), *d_off = d_nnb + ni,
+        *d_nnid = d_off + ni, *d_total = d_nnid + ni;
+    int *d_lst = (int *)(S.d_res.p + off_l);
     launch_zero_int(S.st, d_total, 1);
     const double *dx = S.d_in.p, *dv = dx + 3 * (size_t)ni,
                  *dh = dx + 6 * (size_t)ni, *dt = dx + 7 * (size_t)ni;
     enqueue_regf(ni, dx, dv, dh, dt, d_acc, d_jrk, d_pot, nnbmax, m_flag,
-                 d_nnb, d_nnid, S.d_list.p, d_off, d_total, list_cap);
+                 d_nnb, d_nnid, d_lst, d_off, d_total, list_cap);
 
-    // Results back: fixed-size outputs plus a speculative prefix of the
-    // compact list (sized from the previous call); the rest, if any, after
-    // the synchronisation.
+    // Results back in one copy: the fixed-size outputs plus a speculative
+    // prefix of the compact list (sized from the previous call); the rest,
+    // if any, after the synchronisation.
     long long est = std::max<long long>((long long)ni * 64, S.list_est);
     est = std::min(est, list_cap);
-    S.h_list.ensure((size_t)est);
-    HIPCHK(hipMemcpyAsync(S.h_out.p, S.d_out.p, 7 * (size_t)ni * sizeof(double),
-                          hipMemcpyDeviceToHost, S.st));
-    HIPCHK(hipMemcpyAsync(S.h_iout.p, S.d_iout.p,
-                          (3 * (size_t)ni + 1) * sizeof(int),
-                          hipMemcpyDeviceToHost, S.st));
-    HIPCHK(hipMemcpyAsync(S.h_list.p, S.d_list.p, (size_t)est * sizeof(int),
+    S.h_res.ensure(off_l + (size_t)est * sizeof(int));
+    HIPCHK(hipMemcpyAsync(S.h_res.p, S.d_res.p,
+                          off_l + (size_t)est * sizeof(int),
                           hipMemcpyDeviceToHost, S.st));
     err_fetch();
     HIPCHK(hipStreamSynchronize(S.st));
@@ -626,18 +617,19 @@ void do_regf(int ni, const double *h2, const double *dtr, const double (*xi)[3],
     double t1 = wtime();
     S.time_grav += t1 - t0;
 
-    const int *h_nnb = S.h_iout.p, *h_off = h_nnb + ni, *h_nnid = h_off + ni;
-    const long long total = S.h_iout.p[3 * (size_t)ni];
+    const long long total =
+        ((const int *)(S.h_res.p + off_i))[3 * (size_t)ni];
     S.list_est = total + total / 4 + 64;
     if (total > est) {
-        // Grow the pinned buffer, keeping the prefix already copied.
-        HostBuf<int> nb;
-        nb.ensure((size_t)total);
-        memcpy(nb.p, S.h_list.p, (size_t)est * sizeof(int));
-        S.h_list.release();
-        S.h_list = nb;
+        // Grow the pinned buffer, keeping what was already copied.
+        const size_t have = off_l + (size_t)est * sizeof(int);
+        HostBuf<char> nb;
+        nb.ensure(off_l + (size_t)total * sizeof(int));
+        memcpy(nb.p, S.h_res.p, have);
+        S.h_res.release();
+        S.h_res = nb;
         nb.p = nullptr;
-        HIPCHK(hipMemcpyAsync(S.h_list.p + est, S.d_list.p + est,
+        HIPCHK(hipMemcpyAsync(S.h_res.p + have, S.d_res.p + have,
                               (size_t)(total - est) * sizeof(int),
                               hipMemcpyDeviceToHost, S.st));
         HIPCHK(hipStreamSynchronize(S.st));
@@ -645,8 +637,11 @@ void do_regf(int ni, const double *h2, const double *dtr, const double (*xi)[3],
     double t2 = wtime();
     S.time_nb += t2 - t1;
 
-    const double *h_acc = S.h_out.p, *h_jrk = h_acc + 3 * (size_t)ni,
-                 *h_pot = h_acc + 6 * (size_t)ni;
+    const int *h_nnb = (const int *)(S.h_res.p + off_i), *h_off = h_nnb + ni,
+              *h_nnid = h_off + ni;
+    const int *h_lst = (const int *)(S.h_res.p + off_l);
+    const double *h_acc = (const double *)S.h_res.p,
+                 *h_jrk = h_acc + 3 * (size_t)ni, *h_pot = h_acc + 6 * (size_t)ni;
     memcpy(acc, h_acc, 3 * (size_t)ni * sizeof(double));
     memcpy(jrk, h_jrk, 3 * (size_t)ni * sizeof(double));
     memcpy(pot, h_pot, (size_t)ni * sizeof(double));
@@ -655,7 +650,7 @@ void do_regf(int ni, const double *h2, const double *dtr, const double (*xi)[3],
         const int nnb = h_nnb[i];
         row[0] = nnb;
         if (nnb > 0)
-            memcpy(row + 1, S.h_list.p + h_off[i], (size_t)nnb * sizeof(int));
+            memcpy(row + 1, h_lst + h_off[i], (size_t)nnb * sizeof(int));
     }
     if (nnid) memcpy(nnid, h_nnid, (size_t)ni * sizeof(int));
     S.time_out += wtime() - t2;

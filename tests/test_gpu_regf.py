@@ -66,12 +66,15 @@ def test_overflow_codes_and_all_neighbours():
     o = oracle.regf(*args)
     assert (o[3][:, 0] < 0).any() and (o[3][:, 0] >= 0).any()
     assert_parity(g, o, label="overflow", inputs=args[:3] + args[5:7])
-    # Huge radius: every j is a neighbour, the far field is empty.
+    # Huge radius: every j is a neighbour, the far field is empty.  The
+    # lists (700 x 700 entries) outgrow the speculative first copy of the
+    # compact list, so the rest comes back in a second copy.
     big = np.full(c.n, 1e6)
     args = (c.m, c.x, c.v, big, dtr, c.x, c.v, 1001, 1000, 0)
     g = gpu_regf(*args)
     o = oracle.regf(*args)
     assert np.array_equal(g[3][:, 0], o[3][:, 0])
+    assert np.array_equal(g[3][:, 1:c.n + 1], o[3][:, 1:c.n + 1])
     assert np.all(g[3][:, 0] == c.n)
     assert np.all(g[2] == 0.0) and np.all(g[0] == 0.0)
 
