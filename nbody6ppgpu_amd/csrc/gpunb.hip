@@ -245,6 +245,15 @@ void set_device() { HIPCHK(hipSetDevice(S.dev)); }
 // The device error word accumulates until the next synchronisation point
 // (an error raised by an asynchronous send is reported by the following
 // regf); err_check clears it when it reports.
+size_t round_up16(size_t b) { return (b + 15) & ~(size_t)15; }
+
+// Device address of a pinned host buffer (hipHostMalloc memory is mapped).
+void *host_dev_ptr(void *h) {
+    void *d = nullptr;
+    HIPCHK(hipHostGetDevicePointer(&d, h, 0));
+    return d;
+}
+
 void err_fetch() {
     HIPCHK(hipMemcpyAsync(S.h_err.p, S.err.p, sizeof(unsigned),
                           hipMemcpyDeviceToHost, S.st));
@@ -590,7 +599,7 @@ void do_regf(int ni, const double *h2, const double *dtr, const double (*xi)[3],
     const size_t off_i = 7 * (size_t)ni * sizeof(double);
     const size_t off_l =
         (off_i + (3 * (size_t)ni + 1) * sizeof(int) + 255) & ~(size_t)255;
-    S.d_res.ensure(off_l + (size_t)list_cap * sizeof(int));
+    S.d_res.ensure(round_up16(off_l + (size_t)list_cap * sizeof(int)));
     double *d_acc = (double *)S.d_res.p, *d_jrk = d_acc + 3 * (size_t)ni,
            *d_pot = d_acc + 6 * (size_t)ni;
     int *d_nnb = (int *)(S.d_res.p + off_i), *d_off = d_nnb + ni,
@@ -607,11 +616,10 @@ void do_regf(int ni, const double *h2, const double *dtr, const double (*xi)[3],
     // if any, after the synchronisation.
     long long est = std::max<long long>((long long)ni * 64, S.list_est);
     est = std::min(est, list_cap);
-    S.h_res.ensure(off_l + (size_t)est * sizeof(int));
-    HIPCHK(hipMemcpyAsync(S.h_res.p, S.d_res.p,
-                          off_l + (size_t)est * sizeof(int),
-                          hipMemcpyDeviceToHost, S.st));
-    err_fetch();
+    const size_t first = round_up16(off_l + (size_t)est * sizeof(int));
+    S.h_res.ensure(first);
+    launch_copy_to_host(S.st, S.d_res.p, host_dev_ptr(S.h_res.p), first, S.err.p,
+                        (unsigned *)host_dev_ptr(S.h_err.p));
     HIPCHK(hipStreamSynchronize(S.st));
     err_check();
     double t1 = wtime();

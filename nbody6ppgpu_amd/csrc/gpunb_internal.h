@@ -126,6 +126,10 @@ void launch_regf_pass1(hipStream_t st, const RegfLaunch &L);
 void launch_regf_pass2(hipStream_t st, const RegfLaunch &L);
 int pass2_ipb(int ni);   // pass-2 i per workgroup for a cluster of ni
 void launch_zero_int(hipStream_t st, int *p, int n);
+// bytes (a multiple of 16, both pointers 16-byte aligned) from device memory
+// to a mapped pinned host buffer, and *err to *herr (host-mapped).
+void launch_copy_to_host(hipStream_t st, const void *src, void *dst,
+                         size_t bytes, const unsigned *err, unsigned *herr);
 void launch_pot(hipStream_t st, const PRec *p, int n, int ntile, int i0,
                 int ni, int nslice, double *part, double *pot);
 void launch_pack_p(hipStream_t st, const double *m, const double *x, int n,

@@ -278,6 +278,20 @@ __global__ __launch_bounds__(256) void k_jget(int n, const JRec *__restrict__ jp
     v[3 * (size_t)j + 2] = r.vz;
 }
 
+// Device buffer -> mapped pinned host buffer, 16 bytes per lane (each
+// wavefront writes whole 1 KiB runs across the host link), plus the error
+// word: the results of a host-array call come back without a copy-engine
+// transfer, in stream order.
+__global__ __launch_bounds__(256) void k_copy_to_host(
+    const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n16,
+    const unsigned *__restrict__ err, unsigned *__restrict__ herr) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n16;
+         k += stride)
+        dst[k] = src[k];
+    if (blockIdx.x == 0 && threadIdx.x == 0) *herr = *err;
+}
+
 __global__ void k_zero_int(int *p, int n) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n) p[k] = 0;
@@ -728,6 +742,7 @@ __global__ __launch_bounds__(64) void k_regf_pass1_f64(
 // ---------------------------------------------------------------------------
 
 constexpr int kP2I = 32;
+constexpr int kP2U = 4;   // 64-slice chunks per pass-2 load batch
 
 __global__ __launch_bounds__(256) void k_regf_pass2(
     const JRec *__restrict__ jp_all, const ClusterDesc *__restrict__ desc,
@@ -763,21 +778,42 @@ __global__ __launch_bounds__(256) void k_regf_pass2(
         const int i = i0 + r;
         int tot = 0;
         if (i < ni) {
+            // kP2U chunks of 64 slices per batch: their partials and counts
+            // are loaded together, then added in slice order per lane.
             double sum[7] = {0, 0, 0, 0, 0, 0, 0};
-            for (int q = lane; q < nslice; q += 64) {
-                const double *p = part + ((size_t)i * nslice + q) * 8;
+            const int *ci = cnt + (size_t)i * nslice;
+            int ct = 0;
+            for (int b0 = 0; b0 < nslice; b0 += 64 * kP2U) {
+                double v[kP2U][7];
+                int c[kP2U];
 #pragma unroll
-                for (int k = 0; k < 7; ++k) sum[k] += p[k];
+                for (int u = 0; u < kP2U; ++u) {
+                    const int q = b0 + 64 * u + lane;
+                    if (q < nslice) {
+                        const double *p = part + ((size_t)i * nslice + q) * 8;
+#pragma unroll
+                        for (int k = 0; k < 7; ++k) v[u][k] = p[k];
+                        c[u] = ci[q];
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < 7; ++k) v[u][k] = 0.0;
+                        c[u] = 0;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < kP2U; ++u) {
+                    if (b0 + 64 * u + lane < nslice) {
+#pragma unroll
+                        for (int k = 0; k < 7; ++k) sum[k] += v[u][k];
+                    }
+                    ct += c[u];
+                }
             }
 #pragma unroll
             for (int k = 0; k < 7; ++k)
                 for (int o = 32; o >= 1; o >>= 1)
                     sum[k] += __shfl_xor(sum[k], o);
-            const int *ci = cnt + (size_t)i * nslice;
-            for (int q0 = 0; q0 < nslice; q0 += 64) {
-                const int q = q0 + lane;
-                tot += wave_sum(q < nslice ? ci[q] : 0);
-            }
+            tot = wave_sum(ct);
             if (lane == 0) {
                 acc[3 * (size_t)i + 0] = sum[0];
                 acc[3 * (size_t)i + 1] = sum[1];
@@ -822,17 +858,47 @@ __global__ __launch_bounds__(256) void k_regf_pass2(
             double br2 = INFINITY;
             int bj = INT_MAX;
             int base = o;
-            for (int q0 = 0; q0 < nslice; q0 += 64) {
-                const int q = q0 + lane;
-                const int cq = q < nslice ? ci[q] : 0;
-                const int incl = wave_incl_scan(cq, lane);
-                if (cq > 0) {
-                    const int js = (int)(((long long)ntile * q) / nslice) * kTile;
-                    const uint16_t *sq = seg + ((size_t)i * nslice + q) * cap;
-                    int *dst = list + base + (incl - cq);
-                    for (int k = 0; k < cq; ++k) {
-                        const int j = js + (int)sq[k];
-                        dst[k] = j;
+            // kP2U chunks of 64 slices per batch: counts loaded together,
+            // list offsets by wave scans, then the k-th entry of every
+            // chunk's segment (and its j record) loaded together.  The
+            // nearest neighbour is a strict (r2, j) minimum, so the order of
+            // the updates does not matter.
+            for (int b0 = 0; b0 < nslice; b0 += 64 * kP2U) {
+                int cq[kP2U], d0[kP2U];
+#pragma unroll
+                for (int u = 0; u < kP2U; ++u) {
+                    const int q = b0 + 64 * u + lane;
+                    cq[u] = q < nslice ? ci[q] : 0;
+                }
+                int mc = 0;
+#pragma unroll
+                for (int u = 0; u < kP2U; ++u) {
+                    const int incl = wave_incl_scan(cq[u], lane);
+                    d0[u] = base + (incl - cq[u]);
+                    base += __shfl(incl, 63);
+                    mc = cq[u] > mc ? cq[u] : mc;
+                }
+                for (int o2 = 32; o2 >= 1; o2 >>= 1) {
+                    const int t = __shfl_xor(mc, o2);
+                    mc = t > mc ? t : mc;
+                }
+                for (int k = 0; k < mc; ++k) {
+                    int jv[kP2U];
+#pragma unroll
+                    for (int u = 0; u < kP2U; ++u) {
+                        jv[u] = -1;
+                        if (k < cq[u]) {
+                            const int q = b0 + 64 * u + lane;
+                            const int js =
+                                (int)(((long long)ntile * q) / nslice) * kTile;
+                            jv[u] = js + (int)seg[((size_t)i * nslice + q) * cap + k];
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < kP2U; ++u) {
+                        if (jv[u] < 0) continue;
+                        const int j = jv[u];
+                        list[d0[u] + k] = j;
                         const JRec &J = jp[j];
                         const double r2 = sq3d(J.x - xa, J.y - ya, J.z - za);
                         if (r2 > 0.0 && (r2 < br2 || (r2 == br2 && j < bj))) {
@@ -841,7 +907,6 @@ __global__ __launch_bounds__(256) void k_regf_pass2(
                         }
                     }
                 }
-                base += __shfl(incl, 63);
             }
             // Lexicographic (r2, j) minimum over the wave.
             for (int w = 32; w >= 1; w >>= 1) {
@@ -995,6 +1060,14 @@ void launch_jget(hipStream_t st, int n, const JRec *jp, double *x, double *v) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_jget, dim3(cdiv(n, 256)), dim3(256), 0, st, n, jp, x,
                        v);
+}
+
+void launch_copy_to_host(hipStream_t st, const void *src, void *dst,
+                         size_t bytes, const unsigned *err, unsigned *herr) {
+    const size_t n16 = bytes / 16;
+    const int nb = (int)std::min<size_t>(1024, std::max<size_t>(1, (n16 + 255) / 256));
+    hipLaunchKernelGGL(k_copy_to_host, dim3(nb), dim3(256), 0, st,
+                       (const uint4 *)src, (uint4 *)dst, n16, err, herr);
 }
 
 void launch_zero_int(hipStream_t st, int *p, int n) {
