@@ -104,6 +104,9 @@ struct State {
     bool ev_send_live = false, ev_jp_live = false, ev_desc_live = false;
     HostBuf<ClusterDesc> h_desc;
     DevBuf<ClusterDesc> d_desc;
+    // What d_desc holds (uploaded by enqueue_regf); an unchanged table is not
+    // uploaded again.
+    std::vector<ClusterDesc> desc_up;
 
     int nbmax = 0;
     int nj = 0;       // current j count used by regf
@@ -372,15 +375,13 @@ void enqueue_regf(int ni, const double *xi, const double *vi, const double *h2,
     const int prec = pick_prec();
     const int ntile = cdiv(S.nj, kTile);
     const int nch = cdiv(ni, kChunk);
-    wait_event(S.ev_desc, S.ev_desc_live);
-    S.h_desc.ensure((size_t)nch);
-    S.d_desc.ensure((size_t)nch);
+    std::vector<ClusterDesc> nd((size_t)nch);
     for (int c = 0; c < nch; ++c) {
         const int c0 = c * kChunk;
         const int nc = std::min(kChunk, ni - c0);
         Decomp d = decompose(nc, ntile, nnbmax, m_flag, group, prec);
         ensure_regf_scratch(nc, d);
-        ClusterDesc &D = S.h_desc.p[c];
+        ClusterDesc &D = nd[c];
         memset(&D, 0, sizeof D);
         D.nj = S.nj;
         D.ntile = ntile;
@@ -391,12 +392,23 @@ void enqueue_regf(int ni, const double *xi, const double *vi, const double *h2,
         D.cap = ntile == 0 ? 1 : d.cap;
         D.ipb = pass2_ipb(nc);
     }
-    HIPCHK(hipMemcpyAsync(S.d_desc.p, S.h_desc.p, nch * sizeof(ClusterDesc),
-                          hipMemcpyHostToDevice, S.st));
-    HIPCHK(hipEventRecord(S.ev_desc, S.st));
-    S.ev_desc_live = true;
+    const size_t had = S.d_desc.n;
+    S.d_desc.ensure((size_t)nch);
+    if (S.d_desc.n != had) S.desc_up.clear();   // new buffer: nothing held
+    if (S.desc_up.size() != nd.size() ||
+        memcmp(S.desc_up.data(), nd.data(), nd.size() * sizeof(ClusterDesc))) {
+        // h_desc may still be read by the previous upload.
+        wait_event(S.ev_desc, S.ev_desc_live);
+        S.h_desc.ensure((size_t)nch);
+        memcpy(S.h_desc.p, nd.data(), nd.size() * sizeof(ClusterDesc));
+        HIPCHK(hipMemcpyAsync(S.d_desc.p, S.h_desc.p, nch * sizeof(ClusterDesc),
+                              hipMemcpyHostToDevice, S.st));
+        HIPCHK(hipEventRecord(S.ev_desc, S.st));
+        S.ev_desc_live = true;
+        S.desc_up = nd;
+    }
     for (int c = 0; c < nch; ++c) {
-        const ClusterDesc &D = S.h_desc.p[c];
+        const ClusterDesc &D = nd[c];
         RegfLaunch L;
         memset(&L, 0, sizeof L);
         L.jp = S.jp.p;
@@ -592,8 +604,6 @@ void do_regf(int ni, const double *h2, const double *dtr, const double (*xi)[3],
     memcpy(S.h_in.p + 3 * (size_t)ni, vi, 3 * (size_t)ni * sizeof(double));
     memcpy(S.h_in.p + 6 * (size_t)ni, h2, (size_t)ni * sizeof(double));
     memcpy(S.h_in.p + 7 * (size_t)ni, dtr, (size_t)ni * sizeof(double));
-    HIPCHK(hipMemcpyAsync(S.d_in.p, S.h_in.p, n8 * sizeof(double),
-                          hipMemcpyHostToDevice, S.st));
 
     const long long list_cap = std::max<long long>(1, (long long)ni * nnbmax);
     const size_t off_i = 7 * (size_t)ni * sizeof(double);
@@ -605,7 +615,9 @@ void do_regf(int ni, const double *h2, const double *dtr, const double (*xi)[3],
     int *d_nnb = (int *)(S.d_res.p + off_i), *d_off = d_nnb + ni,
         *d_nnid = d_off + ni, *d_total = d_nnid + ni;
     int *d_lst = (int *)(S.d_res.p + off_l);
-    launch_zero_int(S.st, d_total, 1);
+    // i block up (64 ni bytes) and the list cursor cleared, in stream order.
+    launch_copy_from_host(S.st, host_dev_ptr(S.h_in.p), S.d_in.p,
+                          n8 * sizeof(double), d_total);
     const double *dx = S.d_in.p, *dv = dx + 3 * (size_t)ni,
                  *dh = dx + 6 * (size_t)ni, *dt = dx + 7 * (size_t)ni;
     enqueue_regf(ni, dx, dv, dh, dt, d_acc, d_jrk, d_pot, nnbmax, m_flag,

@@ -130,6 +130,10 @@ void launch_zero_int(hipStream_t st, int *p, int n);
 // to a mapped pinned host buffer, and *err to *herr (host-mapped).
 void launch_copy_to_host(hipStream_t st, const void *src, void *dst,
                          size_t bytes, const unsigned *err, unsigned *herr);
+// bytes (a multiple of 16, both pointers 16-byte aligned) from a mapped
+// pinned host buffer to device memory, and *zero = 0.
+void launch_copy_from_host(hipStream_t st, const void *src, void *dst,
+                           size_t bytes, int *zero);
 void launch_pot(hipStream_t st, const PRec *p, int n, int ntile, int i0,
                 int ni, int nslice, double *part, double *pot);
 void launch_pack_p(hipStream_t st, const double *m, const double *x, int n,

@@ -292,6 +292,19 @@ __global__ __launch_bounds__(256) void k_copy_to_host(
     if (blockIdx.x == 0 && threadIdx.x == 0) *herr = *err;
 }
 
+// Mapped pinned host buffer -> device buffer, 16 bytes per lane, and
+// *zero = 0 (the list cursor of the call): the inputs of a host-array call
+// go up in stream order, without a copy-engine transfer.
+__global__ __launch_bounds__(256) void k_copy_from_host(
+    const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n16,
+    int *__restrict__ zero) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n16;
+         k += stride)
+        dst[k] = src[k];
+    if (blockIdx.x == 0 && threadIdx.x == 0) *zero = 0;
+}
+
 __global__ void k_zero_int(int *p, int n) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n) p[k] = 0;
@@ -1068,6 +1081,14 @@ void launch_copy_to_host(hipStream_t st, const void *src, void *dst,
     const int nb = (int)std::min<size_t>(1024, std::max<size_t>(1, (n16 + 255) / 256));
     hipLaunchKernelGGL(k_copy_to_host, dim3(nb), dim3(256), 0, st,
                        (const uint4 *)src, (uint4 *)dst, n16, err, herr);
+}
+
+void launch_copy_from_host(hipStream_t st, const void *src, void *dst,
+                           size_t bytes, int *zero) {
+    const size_t n16 = bytes / 16;
+    const int nb = (int)std::min<size_t>(1024, std::max<size_t>(1, (n16 + 255) / 256));
+    hipLaunchKernelGGL(k_copy_from_host, dim3(nb), dim3(256), 0, st,
+                       (const uint4 *)src, (uint4 *)dst, n16, zero);
 }
 
 void launch_zero_int(hipStream_t st, int *p, int n) {
