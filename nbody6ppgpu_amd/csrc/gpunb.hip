@@ -343,7 +343,12 @@ Decomp decompose(int ni_c, int ntile, int nnbmax, int m_flag, int group,
     return d;
 }
 
-constexpr int kChunk = 32768;  // i-particles per pass-1/pass-2 launch pair
+// i-particles per pass-1/pass-2 launch pair (GPUNB_CHUNK overrides).
+int pick_chunk() {
+    const char *c = getenv("GPUNB_CHUNK");
+    if (c && atoi(c) >= 64) return atoi(c);
+    return 32768;
+}
 
 // Next free timing event (created on first use, reused after a read).
 int timing_event() {
@@ -366,7 +371,7 @@ void ensure_regf_scratch(int ni_c, const Decomp &d) {
 void wait_event(hipEvent_t ev, bool &live);
 
 // Enqueue the regular-force kernels for device-resident i data: one launch
-// pair per chunk of kChunk i, each a one-cluster table.
+// pair per chunk of pick_chunk() i, each a one-cluster table.
 void enqueue_regf(int ni, const double *xi, const double *vi, const double *h2,
                   const double *dtr, double *acc, double *jrk, double *pot,
                   int nnbmax, int m_flag, int *nnb, int *nnid, int *list,
@@ -374,11 +379,12 @@ void enqueue_regf(int ni, const double *xi, const double *vi, const double *h2,
     const int group = pick_group();
     const int prec = pick_prec();
     const int ntile = cdiv(S.nj, kTile);
-    const int nch = cdiv(ni, kChunk);
+    const int chunk = pick_chunk();
+    const int nch = cdiv(ni, chunk);
     std::vector<ClusterDesc> nd((size_t)nch);
     for (int c = 0; c < nch; ++c) {
-        const int c0 = c * kChunk;
-        const int nc = std::min(kChunk, ni - c0);
+        const int c0 = c * chunk;
+        const int nc = std::min(chunk, ni - c0);
         Decomp d = decompose(nc, ntile, nnbmax, m_flag, group, prec);
         ensure_regf_scratch(nc, d);
         ClusterDesc &D = nd[c];
