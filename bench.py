@@ -182,6 +182,120 @@ def regcor_bench(c, h2, stepr, reps=5, cpu=True):
     return out
 
 
+def irr_bench(c, h2, stepr, reps=5, cpu=True, min_s=5.0):
+    """Irregular force through the irr_simd_* drop-in for every body of the
+    King block: lists from gpunb_regf_dev at RS, predictor records x0, v0,
+    a/2, jerk/6 of that regular force, all bodies active at t = 1/64 past
+    t0.  The first call uploads the records and lists; each timed call is
+    one irr_simd_firr_vec_ of all bodies, timed around the C entry point
+    (index upload, kernel, result copy).  The reference's irr.avx.cpp
+    (oracle/_ref, OpenMP over the host cores) on the same state is the
+    baseline, and its forces and nearest neighbours are compared.  Expects
+    the engine open with the bodies sent."""
+    from nbody6ppgpu_amd import gpunb, regcor
+    from nbody6ppgpu_amd.hiprt import DeviceArray
+    n = c.n
+    lmax, nnbmax = 600, 550
+    lib = gpunb.load()
+    d = {k: DeviceArray.from_numpy(a) for k, a in
+         (("x", c.x), ("v", c.v), ("dtr", stepr), ("h2", h2))}
+    o = {k: DeviceArray(sh, dt) for k, sh, dt in
+         (("acc", (n, 3), np.float64), ("jrk", (n, 3), np.float64),
+          ("pot", (n,), np.float64), ("nnb", (n,), np.int32),
+          ("off", (n,), np.int32), ("lst", (n * nnbmax,), np.int32),
+          ("tot", (1,), np.int32))}
+    rc = lib.gpunb_regf_dev(n, d["h2"].ptr, d["dtr"].ptr, d["x"].ptr,
+                            d["v"].ptr, o["acc"].ptr, o["jrk"].ptr,
+                            o["pot"].ptr, nnbmax, 0, o["nnb"].ptr, 0,
+                            o["lst"].ptr, o["off"].ptr, o["tot"].ptr,
+                            n * nnbmax)
+    if rc != 0 or lib.gpunb_sync() != 0:
+        gpunb._check()
+    rows = regcor.compact_to_rows(o["nnb"].numpy(), o["off"].numpy(),
+                                  o["lst"].numpy(), lmax)
+    acc0, jrk0 = o["acc"].numpy(), o["jrk"].numpy()
+    lists = [rows[i, 1:1 + rows[i, 0]] for i in range(n)]
+    pairs = int(rows[:, 0].sum())
+    addr = np.arange(1, n + 1, dtype=np.int32)
+    t0 = np.zeros(n)
+    t = 1.0 / 64
+    f, fd = 0.5 * acc0, jrk0 / 6.0
+    gpunb.irr_open(n + 10, lmax)
+    gpunb.irr_set_jp(addr, c.x, c.v, f, fd, c.m, t0)
+    gpunb.irr_set_lists(addr, lists)
+    acc, jrk, nn = gpunb.irr_firr(t, addr)
+    times, kms = [], []
+    for _ in range(reps):
+        t1 = time.perf_counter()
+        gpunb.irr_firr(t, addr)
+        times.append(time.perf_counter() - t1)
+        kms.append(gpunb.irr_kernel_ms())
+    gpunb.irr_close()
+    ms = float(np.median(times)) * 1e3
+    kms_med = float(np.median(kms))
+    # Algorithmic bytes of the call's kernels.  With more than three pairs
+    # per slot (slots = n + 10 here) the library predicts every slot once
+    # (128 B read, 64 B written per slot) and the force kernel reads per
+    # body its index, predicted record (64 B), list count and outputs
+    # (6 doubles + the nearest index), per pair the member's index and
+    # predicted record (4 + 64 B); otherwise 128-byte records per member.
+    nslot = n + 10
+    if pairs > 3 * nslot:
+        abytes = nslot * (128 + 64) + n * (4 + 64 + 4 + 52) + pairs * (4 + 64)
+    else:
+        abytes = n * (4 + 128 + 4 + 52) + pairs * (4 + 128)
+    leg = {"workload": "King W0=6 N=131072: every body active, its "
+                       "gpunb_regf_dev list at RS, records predicted to "
+                       "t0 + 1/64",
+           "bodies": n, "pairs_per_call": pairs,
+           "ms_per_call": ms, "ms_per_call_all": [x * 1e3 for x in times],
+           "pairs_per_s": pairs / (ms * 1e-3),
+           "bodies_per_s": n / (ms * 1e-3),
+           "note": "ms_per_call is timed around irr_simd_firr_vec_: index "
+                   "upload, k_irr_force and the result copy",
+           "kernel_ms": kms_med, "kernel_ms_all": kms,
+           "kernel_pairs_per_s": pairs / (kms_med * 1e-3),
+           "algorithmic_bytes": abytes,
+           "achieved_GBps": abytes / (kms_med * 1e-3) / 1e9,
+           "peak_GBps": 8000.0}
+    if cpu:
+        from oracle import ref
+        if ref.available("irr_avx"):
+            threads = int(os.environ.get("OMP_NUM_THREADS",
+                                         os.cpu_count() or 1))
+            R = ref.IrrLib("irr_avx", n + 10, lmax + 1)
+            R.set_jp(t0, c.x, c.v, f, fd, c.m)
+            for i in range(n):
+                R.set_list(i, lists[i] - 1)
+            ra, rj, rn = R.firr(t, np.arange(n))
+            calls = 0
+            t1 = time.perf_counter()
+            while True:
+                R.firr(t, np.arange(n))
+                calls += 1
+                if time.perf_counter() - t1 >= min_s:
+                    break
+            dt = (time.perf_counter() - t1) / calls
+            R.close()
+            rel = np.linalg.norm(ra - acc, axis=1) / np.linalg.norm(acc, axis=1)
+            relj = np.linalg.norm(rj - jrk, axis=1) / np.linalg.norm(jrk, axis=1)
+            leg["vs_reference"] = {
+                "acc_rel_median": float(np.median(rel)),
+                "jrk_rel_median": float(np.median(relj)),
+                "nearest_equal_frac": float(np.mean(rn == nn))}
+            if not np.median(rel) < 1e-5:
+                raise RuntimeError(f"irregular force: median relative "
+                                   f"difference {np.median(rel)!r} from the "
+                                   f"reference library")
+            leg["cpu_baseline"] = {
+                "value": pairs / dt, "unit": "pair interactions/s",
+                "cores": threads, "kind": "reference",
+                "sample": f"reference irr.avx.cpp irr_simd_firr_vec_ of all "
+                          f"{n} bodies, {calls} calls in {dt * calls:.1f} s, "
+                          f"OMP_NUM_THREADS={threads}"}
+    return leg
+
+
 def energy_bench(reps=5, cpu=True):
     """Energy sum on the device (gpunb_energy_dev): a Plummer system of
     N = 131072 (126976 singles and 2048 KS pairs), neighbour lists within
@@ -298,6 +412,8 @@ def main():
                     help="skip the device regular-corrector leg")
     ap.add_argument("--no-energy", action="store_true",
                     help="skip the device energy-sum leg")
+    ap.add_argument("--no-irr", action="store_true",
+                    help="skip the irregular-force leg")
     ap.add_argument("--no-hermite", action="store_true",
                     help="skip the Hermite/Ahmad-Cohen integration leg")
     ap.add_argument("--no-ensemble", action="store_true",
@@ -432,6 +548,11 @@ def main():
     if not args.no_regcor:
         log("bench: regular corrector (gpunb_regcor_dev), all bodies")
         regcor_leg = regcor_bench(c, h2, stepr, cpu=not args.no_cpu)
+    # Irregular force of every body through the irr_simd_* drop-in.
+    irr_leg = None
+    if not args.no_irr:
+        log("bench: irregular force (irr_simd_firr_vec_), all bodies")
+        irr_leg = irr_bench(c, h2, stepr, cpu=not args.no_cpu)
     gpunb.gpunb_close()
 
     # Potential-energy sum of the energy check with KS pairs on the device.
@@ -577,6 +698,7 @@ def main():
         "cpu_baseline": cpu,
         "abi_ni1024_pairs_per_s": abi_rate,
         "regcor": regcor_leg,
+        "irr": irr_leg,
         "energy": energy_leg,
         "hermite_1tnb": hermite,
         "ensemble": ensemble,

@@ -237,6 +237,11 @@ int gpunb_irr_set_jp(int n, const int *addr, const double *x, const double *v,
                      const double *t0);
 int gpunb_irr_set_lists(int n, const int *addr, const int *nnb,
                         const int *members);
+/* Diagnostic, no reference counterpart (the reference's irr_simd_profile_
+ * reports host wall time only): duration in ms of the force kernel of the
+ * last irr_simd_firr_vec_ with ni > 0, measured with HIP events on the
+ * library's stream; 0 before the first such call after irr_simd_open_. */
+double gpunb_irr_kernel_ms(void);
 
 /* ------------------------------------------------------------------------ */
 /* Learned neighbour radius (no reference counterpart: the reference sets RS */

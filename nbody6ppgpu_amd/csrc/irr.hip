@@ -13,7 +13,9 @@
 // k_irr_force: one wavefront per active particle.  Every lane predicts its
 // list members with the FP64 polynomial of xbpredall.f:18-26 (the same
 // association as the oracle, no contraction), evaluates the pair in FP64 and
-// accumulates in FP64; the wave then reduces in a fixed order.  The nearest
+// accumulates in FP64; the wave then reduces in a fixed order.  When a call
+// has more than three pairs per slot, k_irr_predict first predicts every
+// slot once with the same function, and the members are read predicted.  The nearest
 // member is the smallest r^2, ties to the smaller index (the reference's
 // packed (r^2, index) minimum, irr.avx.cpp:317-320), returned 1-based, 0 for
 // an empty list.  The reference evaluates the pairs in FP32 on double-single
@@ -71,16 +73,50 @@ __device__ inline void predict(const IRec &r, double t, double *x, double *v) {
 
 constexpr int kWavesPerBlock = 4;
 
+// A particle predicted to the call's time: position, velocity, mass.
+struct alignas(64) PRec {
+    double x[3], v[3], m, pad;
+};
+static_assert(sizeof(PRec) == 64, "PRec must be 64 bytes");
+
+// Every slot predicted once (the same function and operation order as the
+// per-member prediction of k_irr_force<false>, so the values are the same).
+__global__ __launch_bounds__(256) void k_irr_predict(int n, double t,
+                                                     const IRec *__restrict__ jp,
+                                                     PRec *__restrict__ pj) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const IRec &r = jp[k];
+    PRec o;
+    predict(r, t, o.x, o.v);
+    o.m = r.m;
+    o.pad = 0.0;
+    pj[k] = o;
+}
+
+// PRE: members read from the predicted records pj (64 B each) instead of
+// being predicted from jp (128 B each) member by member.
+template <bool PRE>
 __global__ __launch_bounds__(64 * kWavesPerBlock) void k_irr_force(
     int ni, const int *__restrict__ idx, double t, const IRec *__restrict__ jp,
-    const int *__restrict__ list, int lstride, double *__restrict__ acc,
-    double *__restrict__ jrk, int *__restrict__ nnbid) {
+    const PRec *__restrict__ pj, const int *__restrict__ list, int lstride,
+    double *__restrict__ acc, double *__restrict__ jrk,
+    int *__restrict__ nnbid) {
     const int lane = threadIdx.x & 63;
     const int q = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
     if (q >= ni) return;                  // uniform per wavefront
     const int i = idx[q];
     double xi[3], vi[3];
-    predict(jp[i], t, xi, vi);
+    if (PRE) {
+        const PRec &p = pj[i];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            xi[c] = p.x[c];
+            vi[c] = p.v[c];
+        }
+    } else {
+        predict(jp[i], t, xi, vi);
+    }
     const int *row = list + (size_t)i * lstride;
     const int nnb = row[0];
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, j0 = 0.0, j1 = 0.0, j2 = 0.0;
@@ -88,8 +124,19 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void k_irr_force(
     int bestj = INT_MAX;
     for (int k = lane; k < nnb; k += 64) {
         const int j = row[1 + k];
-        double xj[3], vj[3];
-        predict(jp[j], t, xj, vj);
+        double xj[3], vj[3], mj;
+        if (PRE) {
+            const PRec &p = pj[j];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                xj[c] = p.x[c];
+                vj[c] = p.v[c];
+            }
+            mj = p.m;
+        } else {
+            predict(jp[j], t, xj, vj);
+            mj = jp[j].m;
+        }
         const double dx = xj[0] - xi[0], dy = xj[1] - xi[1], dz = xj[2] - xi[2];
         const double dvx = vj[0] - vi[0], dvy = vj[1] - vi[1], dvz = vj[2] - vi[2];
         double r2 = dx * dx;
@@ -102,7 +149,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void k_irr_force(
         if (r2 == 0.0) continue;
         const double rinv = 1.0 / sqrt(r2);
         const double rinv2 = rinv * rinv;
-        const double mr3 = jp[j].m * rinv2 * rinv;
+        const double mr3 = mj * rinv2 * rinv;
         double rv = dx * dvx;
         rv = rv + dy * dvy;
         rv = rv + dz * dvz;
@@ -203,6 +250,7 @@ struct State {
     int nmax = 0, lstride = 0;
     hipStream_t st = nullptr;
     DBuf<IRec> jp;
+    DBuf<PRec> pj;                            // slots predicted to a call's t
     DBuf<int> list;
     // staging (host), latest data per slot wins
     std::vector<IRec> jp_rec;
@@ -220,6 +268,9 @@ struct State {
     // profile
     long long ncall = 0, nsteps = 0, ninter = 0;
     double t_force = 0.0;
+    // k_irr_force of the last call, timed on the stream
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_kernel_ms = 0.0;
 };
 State I;
 
@@ -339,6 +390,9 @@ void do_open(int nmax, int lmax, int rank) {
     IREQ(nmax > 0, "irr_simd_open: nmax must be > 0");
     IREQ(lmax >= 1, "irr_simd_open: lmax must be >= 1");
     ICHK(hipStreamCreateWithFlags(&I.st, hipStreamNonBlocking));
+    ICHK(hipEventCreate(&I.ev0));
+    ICHK(hipEventCreate(&I.ev1));
+    I.last_kernel_ms = 0.0;
     I.nmax = nmax;
     I.lstride = lmax;
     I.jp.ensure((size_t)nmax);
@@ -370,6 +424,7 @@ void do_close(int rank) {
     }
     ICHK(hipStreamSynchronize(I.st));
     I.jp.release();
+    I.pj.release();
     I.list.release();
     I.d_rec.release();
     I.d_int.release();
@@ -379,6 +434,9 @@ void do_close(int rank) {
     I.h_int.release();
     I.h_out.release();
     I.h_nn.release();
+    (void)hipEventDestroy(I.ev0);
+    (void)hipEventDestroy(I.ev1);
+    I.ev0 = I.ev1 = nullptr;
     (void)hipStreamDestroy(I.st);
     I.st = nullptr;
     I.open = false;
@@ -444,11 +502,28 @@ void flush_and_force(double t, int ni, const int *addr, double *acc,
         I.d_nn.ensure((size_t)ni);
         I.h_out.ensure((size_t)ni * 6);
         I.h_nn.ensure((size_t)ni);
-        hipLaunchKernelGGL(k_irr_force,
-                           dim3((unsigned)((ni + kWavesPerBlock - 1) / kWavesPerBlock)),
-                           dim3(64 * kWavesPerBlock), 0, I.st, ni,
-                           I.d_int.p + o_idx, t, I.jp.p, I.list.p, I.lstride,
-                           I.d_out.p, I.d_out.p + 3 * (size_t)ni, I.d_nn.p);
+        ICHK(hipEventRecord(I.ev0, I.st));
+        // Predict every slot once when the call's pairs outnumber the slots
+        // threefold: each member then costs a 64-byte read instead of a
+        // 128-byte read and its prediction.
+        const bool pre = pairs > 3LL * I.nmax;
+        if (pre) {
+            I.pj.ensure((size_t)I.nmax);
+            hipLaunchKernelGGL(k_irr_predict, dim3((unsigned)((I.nmax + 255) / 256)),
+                               dim3(256), 0, I.st, I.nmax, t, I.jp.p, I.pj.p);
+        }
+        const dim3 grid((unsigned)((ni + kWavesPerBlock - 1) / kWavesPerBlock));
+        if (pre)
+            hipLaunchKernelGGL(k_irr_force<true>, grid, dim3(64 * kWavesPerBlock),
+                               0, I.st, ni, I.d_int.p + o_idx, t, I.jp.p,
+                               I.pj.p, I.list.p, I.lstride, I.d_out.p,
+                               I.d_out.p + 3 * (size_t)ni, I.d_nn.p);
+        else
+            hipLaunchKernelGGL(k_irr_force<false>, grid, dim3(64 * kWavesPerBlock),
+                               0, I.st, ni, I.d_int.p + o_idx, t, I.jp.p,
+                               (const PRec *)nullptr, I.list.p, I.lstride,
+                               I.d_out.p, I.d_out.p + 3 * (size_t)ni, I.d_nn.p);
+        ICHK(hipEventRecord(I.ev1, I.st));
         ICHK(hipMemcpyAsync(I.h_out.p, I.d_out.p, (size_t)ni * 6 * sizeof(double),
                             hipMemcpyDeviceToHost, I.st));
         ICHK(hipMemcpyAsync(I.h_nn.p, I.d_nn.p, (size_t)ni * sizeof(int),
@@ -458,6 +533,9 @@ void flush_and_force(double t, int ni, const int *addr, double *acc,
     // The pinned staging buffers are reused by the next call.
     if (nint || nj) ICHK(hipStreamSynchronize(I.st));
     if (ni > 0) {
+        float ms = 0.0f;
+        ICHK(hipEventElapsedTime(&ms, I.ev0, I.ev1));
+        I.last_kernel_ms = ms;
         memcpy(acc, I.h_out.p, (size_t)ni * 3 * sizeof(double));
         memcpy(jrk, I.h_out.p + 3 * (size_t)ni, (size_t)ni * 3 * sizeof(double));
         if (nnbid) memcpy(nnbid, I.h_nn.p, (size_t)ni * sizeof(int));
@@ -543,6 +621,8 @@ IRR_EXPORT int gpunb_irr_set_jp(int n, const int *addr, const double *x,
                      f + 3 * (size_t)q, fd + 3 * (size_t)q, m[q], t0[q]);
     });
 }
+
+IRR_EXPORT double gpunb_irr_kernel_ms(void) { return I.last_kernel_ms; }
 
 IRR_EXPORT int gpunb_irr_set_lists(int n, const int *addr, const int *nnb,
                                    const int *members) {

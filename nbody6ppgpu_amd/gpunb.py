@@ -160,6 +160,7 @@ def _declare(lib: ctypes.CDLL) -> None:
         "gpunb_irr_set_jp": (ctypes.c_int, [ctypes.c_int, _ip, _dp, _dp, _dp,
                                             _dp, _dp, _dp]),
         "gpunb_irr_set_lists": (ctypes.c_int, [ctypes.c_int, _ip, _ip, _ip]),
+        "gpunb_irr_kernel_ms": (ctypes.c_double, []),
         "gpunb_timing": (ctypes.c_int, [ctypes.c_int]),
         "gpunb_timing_read": (ctypes.c_int,
                               [ctypes.POINTER(ctypes.c_double),
@@ -401,6 +402,11 @@ def irr_set_lists(addr, lists) -> None:
         flat = np.zeros(1, np.int32)
     if load().gpunb_irr_set_lists(a.shape[0], _p(a), _p(nnb), _p(flat)) != 0:
         _check()
+
+
+def irr_kernel_ms() -> float:
+    """Duration (ms) of the force kernel of the last irr_firr with ni > 0."""
+    return float(load().gpunb_irr_kernel_ms())
 
 
 def irr_firr(t: float, addr):

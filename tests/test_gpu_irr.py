@@ -79,13 +79,25 @@ def test_matches_oracle_with_edge_lists():
     try:
         idx = np.concatenate([[5, 6, 8], np.arange(100, n, 7)])
         t = 0.003
+        assert gpunb.irr_kernel_ms() == 0.0       # no force call yet
         got = gpunb.irr_firr(t, idx + 1)
         want = oracle.irr(t, idx, t0, x, v, f, fd, m, lists)
         check(got, want, scales(t, idx, t0, x, v, f, fd, m, lists))
         assert got[2][0] == 0 and not got[0][0].any()
-        # ni = 0 is a no-op
+        k_ms = gpunb.irr_kernel_ms()
+        assert 0.0 < k_ms < 1000.0
+        # This call's pairs (~27k) exceed three times the 4000 slots, so the
+        # slots were predicted once; ten particles (~500 pairs) take the
+        # per-member prediction.  Both give the same values bit for bit.
+        few = gpunb.irr_firr(t, idx[:10] + 1)
+        for g, s in zip(few, got):
+            assert np.array_equal(g, s[:10])
+        k_ms = gpunb.irr_kernel_ms()
+        assert 0.0 < k_ms < 1000.0
+        # ni = 0 is a no-op and keeps the last kernel time
         a, j, nn = gpunb.irr_firr(t, np.zeros(0, np.int32))
         assert a.shape == (0, 3)
+        assert gpunb.irr_kernel_ms() == k_ms
     finally:
         gpunb.irr_close()
 
