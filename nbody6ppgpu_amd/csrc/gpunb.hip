@@ -130,6 +130,16 @@ struct State {
     DevBuf<double> part;
     DevBuf<int> cnt;
     DevBuf<uint16_t> seg;
+    // A call of several chunks runs pass 2 of chunk c on st2 while pass 1 of
+    // chunk c + 1 runs on st into the other scratch set (part_b / cnt_b /
+    // seg_b for odd chunks).
+    DevBuf<double> part_b;
+    DevBuf<int> cnt_b;
+    DevBuf<uint16_t> seg_b;
+    hipStream_t st2 = nullptr;
+    hipEvent_t ev_p1[2] = {nullptr, nullptr}, ev_p2[2] = {nullptr, nullptr};
+    hipEvent_t ev_p2end = nullptr;
+    bool p2_live[2] = {false, false};
     // gpunb_regf_ results in one buffer, so that one copy brings them back:
     // acc, jrk, pot (7 ni doubles), then nnb, off, nnid, total (3 ni + 1
     // ints), then the compact list from a 256-byte boundary.
@@ -219,6 +229,12 @@ void devinit(int irank) {
     S.rank = irank;
     HIPCHK(hipSetDevice(dev));
     HIPCHK(hipStreamCreateWithFlags(&S.st, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&S.st2, hipStreamNonBlocking));
+    for (int k = 0; k < 2; ++k) {
+        HIPCHK(hipEventCreateWithFlags(&S.ev_p1[k], hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&S.ev_p2[k], hipEventDisableTiming));
+    }
+    HIPCHK(hipEventCreateWithFlags(&S.ev_p2end, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&S.ev_send, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&S.ev_jp, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&S.ev_desc, hipEventDisableTiming));
@@ -360,11 +376,11 @@ int timing_event() {
     return S.ev_used++;
 }
 
-void ensure_regf_scratch(int ni_c, const Decomp &d) {
+void ensure_regf_scratch(int ni_c, const Decomp &d, int set) {
     const size_t pairs = (size_t)round_up(ni_c, 64) * d.nslice;
-    S.part.ensure(pairs * 8);
-    S.cnt.ensure(pairs);
-    S.seg.ensure(pairs * d.cap);
+    (set ? S.part_b : S.part).ensure(pairs * 8);
+    (set ? S.cnt_b : S.cnt).ensure(pairs);
+    (set ? S.seg_b : S.seg).ensure(pairs * d.cap);
 }
 
 // Enqueue the regular-force kernels for device-resident i data.
@@ -386,7 +402,7 @@ void enqueue_regf(int ni, const double *xi, const double *vi, const double *h2,
         const int c0 = c * chunk;
         const int nc = std::min(chunk, ni - c0);
         Decomp d = decompose(nc, ntile, nnbmax, m_flag, group, prec);
-        ensure_regf_scratch(nc, d);
+        ensure_regf_scratch(nc, d, c & 1);
         ClusterDesc &D = nd[c];
         memset(&D, 0, sizeof D);
         D.nj = S.nj;
@@ -423,9 +439,10 @@ void enqueue_regf(int ni, const double *xi, const double *vi, const double *h2,
         L.vi = vi;
         L.h2 = h2;
         L.dtr = dtr;
-        L.part = S.part.p;
-        L.cnt = S.cnt.p;
-        L.seg = S.seg.p;
+        const int b = c & 1;
+        L.part = b ? S.part_b.p : S.part.p;
+        L.cnt = b ? S.cnt_b.p : S.cnt.p;
+        L.seg = b ? S.seg_b.p : S.seg.p;
         L.acc = acc;
         L.jrk = jrk;
         L.pot = pot;
@@ -444,6 +461,9 @@ void enqueue_regf(int ni, const double *xi, const double *vi, const double *h2,
         L.ncl = 1;
         L.ntask = (long long)D.n_iw * D.nslice;
         L.nblk2 = cdiv(D.ni, D.ipb);
+        // Scratch set b was last read by the pass 2 of chunk c - 2.
+        if (nch > 1 && S.p2_live[b])
+            HIPCHK(hipStreamWaitEvent(S.st, S.ev_p2[b], 0));
         int e0 = -1, e1 = -1;
         if (S.timing) {
             e0 = timing_event();
@@ -456,8 +476,22 @@ void enqueue_regf(int ni, const double *xi, const double *vi, const double *h2,
             S.ev_pairs.emplace_back(e0, e1);
             S.ev_pairs_work.push_back((long long)D.ni * S.nj);
         }
-        launch_regf_pass2(S.st, L);
+        if (nch > 1) {
+            HIPCHK(hipEventRecord(S.ev_p1[b], S.st));
+            HIPCHK(hipStreamWaitEvent(S.st2, S.ev_p1[b], 0));
+            launch_regf_pass2(S.st2, L);
+            HIPCHK(hipEventRecord(S.ev_p2[b], S.st2));
+            S.p2_live[b] = true;
+        } else {
+            launch_regf_pass2(S.st, L);
+        }
         HIPCHK(hipGetLastError());
+    }
+    if (nch > 1) {
+        // Everything after this call on st (copies, the next predictor,
+        // synchronisations) follows the last pass 2.
+        HIPCHK(hipEventRecord(S.ev_p2end, S.st2));
+        HIPCHK(hipStreamWaitEvent(S.st, S.ev_p2end, 0));
     }
 }
 
@@ -541,6 +575,9 @@ void do_close() {
     S.part.release();
     S.cnt.release();
     S.seg.release();
+    S.part_b.release();
+    S.cnt_b.release();
+    S.seg_b.release();
     S.d_res.release();
     S.h_res.release();
     S.rc_nlist.release();
