@@ -216,3 +216,33 @@ def test_king_131072_invariants_all_particles():
     a = rows[off].astype(np.int64) * n + cols[off]
     b = cols[off].astype(np.int64) * n + rows[off]
     assert np.array_equal(np.sort(a), np.sort(b)), "lists not symmetric"
+
+
+def test_chunked_launches_agree(monkeypatch):
+    """One gpunb_regf_ of 70000 i split into launch pairs of 16384 (5
+    chunks, pass 2 on the second stream over both scratch sets), 32768 (3
+    chunks, the default) and 131072 i (one chunk, one stream): identical
+    lists and nearest neighbours; forces equal up to the FP64 summation
+    order of the slice partials, which the chunk size changes."""
+    c = ics.king(70000, 6.0, seed=221, m=ics.kroupa(70000, seed=222))
+    h2, dtr = setup(c, 48)
+    lmax, nnbmax = 601, 550
+    out = {}
+    gpunb.gpunb_open(c.n + 10)
+    try:
+        gpunb.gpunb_send(c.m, c.x, c.v)
+        for chunk in ("16384", "32768", "131072"):
+            monkeypatch.setenv("GPUNB_CHUNK", chunk)
+            out[chunk] = gpunb.gpunb_regf(h2, dtr, c.x, c.v, lmax, nnbmax, 0,
+                                          nearest=True)
+    finally:
+        gpunb.gpunb_close()
+    ref = out["131072"]
+    assert np.all(ref[3][:, 0] >= 1)
+    for chunk in ("16384", "32768"):
+        g = out[chunk]
+        assert np.array_equal(g[3], ref[3]), chunk
+        assert np.array_equal(g[4], ref[4]), chunk
+        for a, b in zip(g[:3], ref[:3]):
+            scale = np.max(np.abs(b))
+            assert np.max(np.abs(a - b)) <= 1e-12 * scale, chunk
