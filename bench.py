@@ -296,6 +296,105 @@ def irr_bench(c, h2, stepr, reps=5, cpu=True, min_s=5.0):
     return leg
 
 
+def predictor_pot_bench(c, cpu=True, reps=50):
+    """The device predictor (gpunb_predict_, k_predict) and the O(N^2)
+    potential (gpupot_).
+
+    Predictor: synthetic predictor states of N = 131072 and 1048576 bodies
+    sent by gpunb_jpset_, HIP events on the library stream around `reps`
+    back-to-back gpunb_predict_ calls; 176 algorithmic bytes per body (13
+    doubles and the mass read, a 64-byte j record written).  The predicted
+    records of the 131072-body set must equal the oracle's xbpredall.f
+    restatement bit for bit; the oracle (serial C) is the CPU baseline.
+
+    Potential: gpupot_ of all bodies of the King block (host arrays in and
+    out, as energy.F:38 calls it), wall time per call; the reference
+    pot.avx.cpp (oracle/_ref, OpenMP over the host cores) on the same
+    bodies as the baseline and as a check."""
+    from nbody6ppgpu_amd import gpunb
+    from nbody6ppgpu_amd.hiprt import Event
+    leg = {"predict": []}
+    rng = np.random.default_rng(31)
+    for n in (131072, 1048576):
+        x0, v0 = rng.normal(size=(n, 3)), rng.normal(size=(n, 3))
+        f, fd = rng.normal(size=(n, 3)), rng.normal(size=(n, 3))
+        m, t0 = np.full(n, 1.0 / n), rng.uniform(-0.01, 0.0, n)
+        t = 0.001
+        gpunb.gpunb_open(n + 10)
+        try:
+            gpunb.gpunb_jpset(np.arange(1, n + 1), x0, v0, f, fd, m, t0)
+            st = gpunb.gpunb_stream()
+            for _ in range(3):
+                gpunb.gpunb_predict(t)
+            gpunb.gpunb_sync()
+            e0, e1 = Event(), Event()
+            e0.record(st)
+            for _ in range(reps):
+                gpunb.gpunb_predict(t)
+            e1.record(st)
+            gpunb.gpunb_sync()
+            us = e0.elapsed_ms(e1) / reps * 1e3
+            xd, vd = gpunb.gpunb_jpget(n) if n == 131072 else (None, None)
+        finally:
+            gpunb.gpunb_close()
+        row = {"bodies": n, "us_per_predict": us,
+               "achieved_GBps": 176 * n / (us * 1e-6) / 1e9, "peak_GBps": 8000.0}
+        if xd is not None:
+            from oracle import oracle
+            xo, vo = oracle.predict(t, t0, x0, v0, f, fd)
+            row["bit_exact_vs_oracle"] = bool(np.array_equal(xd, xo) and
+                                              np.array_equal(vd, vo))
+            if not row["bit_exact_vs_oracle"]:
+                raise RuntimeError("device predictor differs from the oracle")
+            if cpu:
+                calls, t1 = 0, time.perf_counter()
+                while time.perf_counter() - t1 < 2.0:
+                    oracle.predict(t, t0, x0, v0, f, fd)
+                    calls += 1
+                dt = (time.perf_counter() - t1) / calls
+                row["cpu_baseline"] = {
+                    "value": n / dt, "unit": "bodies/s", "cores": 1,
+                    "kind": "port",
+                    "sample": f"oracle/oracle.c orc_predict of {n} bodies, "
+                              f"{calls} calls, {dt * 1e3:.2f} ms each"}
+        leg["predict"].append(row)
+    n = c.n
+    gpunb.gpunb_open(n + 10)
+    try:
+        phi = gpunb.gpupot(1, n, c.m, c.x)
+        times = []
+        for _ in range(3):
+            t1 = time.perf_counter()
+            gpunb.gpupot(1, n, c.m, c.x)
+            times.append(time.perf_counter() - t1)
+    finally:
+        gpunb.gpunb_close()
+    ms = float(np.median(times)) * 1e3
+    pot = {"workload": f"gpupot_ of all {n} bodies of the King W0=6 block",
+           "ms_per_call": ms, "pairs_per_s": float(n) * n / (ms * 1e-3)}
+    if cpu:
+        from oracle import ref
+        if ref.available("pot_avx"):
+            threads = int(os.environ.get("OMP_NUM_THREADS",
+                                         os.cpu_count() or 1))
+            ni = n
+            t1 = time.perf_counter()
+            pr = ref.pot("pot_avx", c.m, c.x, 1, ni)
+            dt = time.perf_counter() - t1
+            rel = np.abs(pr - phi[:ni]) / np.abs(phi[:ni])
+            pot["vs_reference_rel_max"] = float(rel.max())
+            if not rel.max() < 1e-5:
+                raise RuntimeError(f"gpupot differs from pot.avx.cpp by "
+                                   f"{rel.max()!r}")
+            pot["cpu_baseline"] = {
+                "value": float(ni) * n / dt, "unit": "pair interactions/s",
+                "cores": threads, "kind": "reference",
+                "sample": f"reference pot.avx.cpp gpupot_ of all {n} "
+                          f"bodies, {dt:.2f} s, OMP_NUM_THREADS={threads}"}
+    leg["gpupot"] = pot
+    return leg
+
+
 def energy_bench(reps=5, cpu=True):
     """Energy sum on the device (gpunb_energy_dev): a Plummer system of
     N = 131072 (126976 singles and 2048 KS pairs), neighbour lists within
@@ -414,6 +513,8 @@ def main():
                     help="skip the device energy-sum leg")
     ap.add_argument("--no-irr", action="store_true",
                     help="skip the irregular-force leg")
+    ap.add_argument("--no-predict-pot", action="store_true",
+                    help="skip the predictor and gpupot legs")
     ap.add_argument("--no-hermite", action="store_true",
                     help="skip the Hermite/Ahmad-Cohen integration leg")
     ap.add_argument("--no-ensemble", action="store_true",
@@ -554,6 +655,12 @@ def main():
         log("bench: irregular force (irr_simd_firr_vec_), all bodies")
         irr_leg = irr_bench(c, h2, stepr, cpu=not args.no_cpu)
     gpunb.gpunb_close()
+
+    # Device predictor and the O(N^2) potential.
+    pp_leg = None
+    if not args.no_predict_pot:
+        log("bench: device predictor and gpupot_")
+        pp_leg = predictor_pot_bench(c, cpu=not args.no_cpu)
 
     # Potential-energy sum of the energy check with KS pairs on the device.
     energy_leg = None
@@ -699,6 +806,7 @@ def main():
         "abi_ni1024_pairs_per_s": abi_rate,
         "regcor": regcor_leg,
         "irr": irr_leg,
+        "predictor_pot": pp_leg,
         "energy": energy_leg,
         "hermite_1tnb": hermite,
         "ensemble": ensemble,
