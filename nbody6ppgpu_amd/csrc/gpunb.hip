@@ -314,12 +314,13 @@ double pick_rounds(int ni) {
     const char *r = getenv("GPUNB_WAVE_ROUNDS");
     if (r && atof(r) > 0) return atof(r);
     // Resident-wave capacity multiples per launch, by block size: large
-    // blocks take 4 (smaller tasks even out the tail: N = 131072, 3.26 ms
-    // per launch at 4 against 3.51 at 2), small ones fewer, since every
-    // extra slice is one more FP64 partial per i to write and reduce
-    // (whole step, 1024 i: 0.187 ms at 1, 0.206 at 4; 128 i: 0.108 ms at
-    // 0.5, 0.121 at 4; 4096 i and up: 4 best).
-    return std::min(4.0, std::max(0.5, ni / 1024.0));
+    // blocks take 6 (smaller tasks even out the tail: N = 131072, 3.26 ms
+    // per launch at 4 against 3.51 at 2; with pass 2 overlapping the next
+    // chunk's pass 1, the whole step is about 1% faster at 6 or 8 than at
+    // 4), small ones fewer, since every extra slice is one more FP64
+    // partial per i to write and reduce (whole step, 1024 i: 0.187 ms at
+    // 1, 0.206 at 4; 128 i: 0.108 ms at 0.5, 0.121 at 4).
+    return std::min(6.0, std::max(0.5, ni / 1024.0));
 }
 
 int pick_prec() {
