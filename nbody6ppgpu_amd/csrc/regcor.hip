@@ -131,16 +131,18 @@ __global__ void k_regcor(RegcorLaunch L) {
                 v[u] = k0 + u < cg ? L.lst[og + k0 + u] : 0;
 #pragma unroll
             for (int u = 0; u < kChunk; ++u) {
-                if (k0 + u >= cg) break;
+                // Predicated rather than branched: after the first bad entry
+                // nothing more is written, and the body is rejected below.
                 const int J = v[u] + P.ifirst;
-                if (J == I) continue;
-                if (!in_j(J) || (nnb > 0 && J <= prev) || nnb + 3 > lmax) {
-                    ok = false;
-                    break;
+                const bool live = ok && k0 + u < cg && J != I;
+                const bool good = in_j(J) && !(nnb > 0 && J <= prev) &&
+                                  nnb + 3 <= lmax;
+                if (live && !good) ok = false;
+                if (live && good) {
+                    NL(nnb + 2) = J;
+                    prev = J;
+                    ++nnb;
                 }
-                NL(nnb + 2) = J;
-                prev = J;
-                ++nnb;
             }
         }
         NL(1) = nnb;
@@ -153,12 +155,9 @@ __global__ void k_regcor(RegcorLaunch L) {
 #pragma unroll
             for (int u = 0; u < kChunk; ++u) {
                 const int l = l0 + u;
-                if (l > nnb0 + 1) break;
-                if (!in_j(v[u]) || (l > 2 && v[u] <= prev)) {
-                    ok = false;
-                    break;
-                }
-                prev = v[u];
+                const bool live = l <= nnb0 + 1;
+                if (live && (!in_j(v[u]) || (l > 2 && v[u] <= prev))) ok = false;
+                if (live) prev = v[u];
             }
         }
     }
