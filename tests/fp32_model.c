@@ -1,0 +1,192 @@
+/*
+ * fp32_model.c -- CPU model of the rounding of the regular-force pair
+ * kernel's FP32 path, for the precision study in test_precision_model.py.
+ *
+ * TEST INFRASTRUCTURE ONLY.  It models the engine's arithmetic (not the
+ * reference's) so that the error budget of each design choice can be shown
+ * on the CPU against the FP64 oracle:
+ *
+ *   mode 0  pair differences x_j - x_i in FP64, rounded to FP32
+ *   mode 1  tile-relative differences: j in the given order is cut into
+ *           tiles of 64, C_t = midpoint of the tile's bounding box (positions
+ *           and velocities apart), dx = fl32(C_t - x_i) + fl32(x_j - C_t)
+ *   mode 2  everything in FP64 except 1/r, rounded to FP32 (the floor any
+ *           FP32 pair arithmetic has)
+ *
+ * Then, unless the pair is in the near band, FP32 r2 = |dx|^2 and
+ * dx.dv with fused multiply-adds, 1/r = the correctly rounded FP32 value
+ * moved by -1, 0 or +1 ulp (a deterministic hash of (i, j); the hardware
+ * v_rsq_f32 is specified to 1 ulp), and the FP32 force terms accumulated in
+ * FP32 and folded into FP64 every 64 j.  Near-band pairs -- min(r2, q) <
+ * band * h2 * w_j with q = r2 + 2 dtr dx.dv, w_j = max(1, x^(2/3),
+ * 1.4 x / K), x = m_j / mbar, band = K (times mbar for mass-weighted lists) --
+ * and list neighbours are evaluated in FP64 as the oracle does; in mode 1
+ * the threshold band * h2 is raised per tile to (e_t / 2)^2, e_t = the
+ * largest |fl32(x_j - C_t)| of the tile, as the engine does.  band = 0
+ * switches the band off (only neighbours and r2 == 0 leave the FP32 sums).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+
+static uint32_t hash2(uint32_t a, uint32_t b) {
+    uint32_t h = a * 0x9E3779B1u ^ (b + 0x7F4A7C15u + (a << 6) + (a >> 2));
+    h ^= h >> 15;
+    h *= 0x2C1B3C6Du;
+    h ^= h >> 12;
+    h *= 0x297A2D39u;
+    h ^= h >> 15;
+    return h;
+}
+
+static float rsq_model(float r2, uint32_t i, uint32_t j) {
+    float r = (float)(1.0 / sqrt((double)r2));
+    switch (hash2(i, j) % 3u) {
+    case 1: return nextafterf(r, INFINITY);
+    case 2: return nextafterf(r, 0.0f);
+    default: return r;
+    }
+}
+
+static double sq3(double a, double b, double c) {
+    double s = a * a;
+    s = s + b * b;
+    s = s + c * c;
+    return s;
+}
+
+/* stats[0] = band pairs evaluated in FP64, stats[1] = neighbour pairs. */
+void model_regf(int nj, const double *m, const double *xj, const double *vj,
+                int ni, const double *h2, const double *dtr, const double *xi,
+                const double *vi, int m_flag, int mode, double band,
+                double K, double *acc, double *jrk, double *pot,
+                long long *stats) {
+    double mbar = 0.0;
+    for (int j = 0; j < nj; j++) mbar += m[j];
+    mbar = nj > 0 ? mbar / nj : 1.0;
+    int ntile = (nj + 63) / 64;
+    double *cen = malloc(sizeof(double) * 7 * (size_t)(ntile > 0 ? ntile : 1));
+    for (int t = 0; t < ntile; t++) {
+        double lo[6], hi[6];
+        for (int c = 0; c < 6; c++) {
+            lo[c] = INFINITY;
+            hi[c] = -INFINITY;
+        }
+        for (int j = 64 * t; j < 64 * t + 64 && j < nj; j++)
+            for (int c = 0; c < 6; c++) {
+                double v = c < 3 ? xj[3 * (size_t)j + c] : vj[3 * (size_t)j + c - 3];
+                if (v < lo[c]) lo[c] = v;
+                if (v > hi[c]) hi[c] = v;
+            }
+        double *C = cen + 7 * t;
+        for (int c = 0; c < 6; c++) C[c] = 0.5 * lo[c] + 0.5 * hi[c];
+        double e = 0.0;
+        for (int j = 64 * t; j < 64 * t + 64 && j < nj; j++) {
+            double o2 = 0.0;
+            for (int c = 0; c < 3; c++) {
+                const double o = (float)(xj[3 * (size_t)j + c] - C[c]);
+                o2 += o * o;
+            }
+            if (sqrt(o2) > e) e = sqrt(o2);
+        }
+        C[6] = 0.25 * e * e;
+    }
+    long long s_band = 0, s_nb = 0;
+#pragma omp parallel for schedule(dynamic, 8) reduction(+ : s_band, s_nb)
+    for (int i = 0; i < ni; i++) {
+        const double *x0 = xi + 3 * (size_t)i, *v0 = vi + 3 * (size_t)i;
+        float ax = 0, ay = 0, az = 0, jx = 0, jy = 0, jz = 0, ph = 0;
+        double A[7] = {0, 0, 0, 0, 0, 0, 0};
+        const double Bh = band * h2[i] * (m_flag ? mbar : 1.0);
+        const float dtr2x = 2.0f * (float)dtr[i];
+        for (int j = 0; j < nj; j++) {
+            const double *X = xj + 3 * (size_t)j, *V = vj + 3 * (size_t)j;
+            double d[6];
+            for (int c = 0; c < 3; c++) {
+                d[c] = X[c] - x0[c];
+                d[3 + c] = V[c] - v0[c];
+            }
+            const double r2d = sq3(d[0], d[1], d[2]);
+            const double r2pd = sq3(d[0] + dtr[i] * d[3], d[1] + dtr[i] * d[4],
+                                    d[2] + dtr[i] * d[5]);
+            const double mn = r2d < r2pd ? r2d : r2pd;
+            const double H = m_flag ? m[j] * h2[i] : h2[i];
+            int fp64 = 0;
+            if (mn < H) {
+                s_nb++;
+                fp64 = -1;
+            } else if (r2d == 0.0) {
+                fp64 = -1;
+            }
+            float f[6];
+            if (mode == 1) {
+                const double *C = cen + 7 * (j / 64);
+                for (int c = 0; c < 3; c++) {
+                    f[c] = (float)(C[c] - x0[c]) + (float)(X[c] - C[c]);
+                    f[3 + c] = (float)(C[3 + c] - v0[c]) + (float)(V[c] - C[3 + c]);
+                }
+            } else {
+                for (int c = 0; c < 6; c++) f[c] = (float)d[c];
+            }
+            const float r2 = fmaf(f[2], f[2], fmaf(f[1], f[1], f[0] * f[0]));
+            const float rv = fmaf(f[2], f[5], fmaf(f[1], f[4], f[0] * f[3]));
+            if (fp64 == 0 && band > 0.0) {
+                const double x = m[j] / mbar;
+                double w = 1.0;
+                if (x > 1.0) {
+                    w = cbrt(x * x);
+                    if (1.4 * x / K > w) w = 1.4 * x / K;
+                }
+                const float q = fmaf(dtr2x, rv, r2);
+                const float s = r2 < q ? r2 : q;
+                double T = Bh;
+                if (mode == 1 && cen[7 * (j / 64) + 6] > T) T = cen[7 * (j / 64) + 6];
+                if ((double)s < T * w) fp64 = 1;
+            }
+            if (fp64 < 0) {
+                /* list neighbour or r2 == 0: no force term */
+            } else if (fp64 > 0 || mode == 2) {
+                double rinv = 1.0 / sqrt(r2d);
+                if (mode == 2 && fp64 == 0)
+                    rinv = (double)rsq_model((float)r2d, (uint32_t)i, (uint32_t)j);
+                else
+                    s_band++;
+                const double rinv2 = rinv * rinv;
+                const double mr = m[j] * rinv, mr3 = mr * rinv2;
+                const double rvd = d[0] * d[3] + d[1] * d[4] + d[2] * d[5];
+                const double al = -3.0 * rvd * rinv2;
+                A[6] += mr;
+                for (int c = 0; c < 3; c++) {
+                    A[c] += mr3 * d[c];
+                    A[3 + c] += mr3 * (d[3 + c] + al * d[c]);
+                }
+            } else {
+                const float rinv = rsq_model(r2, (uint32_t)i, (uint32_t)j);
+                const float rinv2 = rinv * rinv;
+                const float mr = (float)m[j] * rinv;
+                const float mr3 = mr * rinv2;
+                const float al = (-3.0f * rv) * rinv2;
+                ph += mr;
+                ax = fmaf(mr3, f[0], ax);
+                ay = fmaf(mr3, f[1], ay);
+                az = fmaf(mr3, f[2], az);
+                jx = fmaf(mr3, fmaf(al, f[0], f[3]), jx);
+                jy = fmaf(mr3, fmaf(al, f[1], f[4]), jy);
+                jz = fmaf(mr3, fmaf(al, f[2], f[5]), jz);
+            }
+            if ((j & 63) == 63 || j == nj - 1) {
+                A[0] += ax; A[1] += ay; A[2] += az;
+                A[3] += jx; A[4] += jy; A[5] += jz; A[6] += ph;
+                ax = ay = az = jx = jy = jz = ph = 0;
+            }
+        }
+        for (int c = 0; c < 3; c++) {
+            acc[3 * (size_t)i + c] = A[c];
+            jrk[3 * (size_t)i + c] = A[3 + c];
+        }
+        pot[i] = A[6];
+    }
+    free(cen);
+    stats[0] = s_band;
+    stats[1] = s_nb;
+}
