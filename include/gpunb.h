@@ -312,8 +312,8 @@ int gpunb_timing_read(double *ms_total, long long *launches,
 /* Regular corrector on the device.  Replaces the host loop                  */
 /*   DO II = 1, NI: CALL REGCOR_GPU(I, x_i, v_i, GPU_A, GPU_JERK, LISTGP)     */
 /* of util_gpu.F:152-180 (REGCOR_GPU: regcor_gpu.F:1-805) for an isolated    */
-/* cluster: KZ(13) = KZ(14) = 0 (no cloud or external force; the KZ(14) <= 2 */
-/* tidal FIDOT term uses tidal4), KZ(37) = KZ(38) = 0 (no CHECKL / FPCORR).  */
+/* cluster: KZ(13) = KZ(14) = KZ(37) = KZ(38) = 0 (no cloud or external      */
+/* force, no CHECKL, no FPCORR; calls with these options are refused).       */
 /* ------------------------------------------------------------------------ */
 
 /* Scalars of the reference's common6.h that REGCOR_GPU reads. */
@@ -338,6 +338,12 @@ typedef struct {
     int lmax;          /* LMAX: row length of LIST (>= nnbmax + 3) */
     int nb_flag;       /* NB_FLAG (KZ(39) = 0 or 2): radii about RDENS */
     int m_flag;        /* M_FLAG (KZ(39) = 2 or 3): mass-weighted lists */
+    /* Options of branches the device corrector does not implement; a call
+     * with any of them nonzero is refused (error, nothing changed):
+     * KZ(13) cloud forces (FCLOUD, regcor_gpu.F:93-96), KZ(14) external
+     * force (XTRNLF, :98-118), KZ(37) neighbour additions (CHECKL,
+     * :255-260), KZ(38) force-polynomial corrections (FPCORR, :584-595). */
+    int kz13, kz14, kz37, kz38;
 } gpunb_regcor_par;
 
 /* Device arrays of the body state, indexed by body I - 1 (vectors as three

@@ -14,6 +14,7 @@ module gpunb_iface
     real(c_double) :: time, rscale, rdens(3), rc2, rsfac, znbmin, smax, &
                       dtmin, etar, smin, bodym, tidal4
     integer(c_int) :: nnbopt, nnbmax, n, ntot, ifirst, lmax, nb_flag, m_flag
+    integer(c_int) :: kz13, kz14, kz37, kz38
   end type
   type, bind(C) :: gpunb_regcor_state
     type(c_ptr) :: x0, x0dot, xn, xndot, fi, fidot, fr, frdot, d0, d0r, &

@@ -16,7 +16,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libgpunb.so")
-SOURCES = ["kernels.hip", "gpunb.hip", "mlp.hip", "irr.hip", "regcor.hip", "energy.hip"]
+SOURCES = ["kernels.hip", "jprep.hip", "gpunb.hip", "mlp.hip", "irr.hip", "regcor.hip",
+           "energy.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("GPUNB_ARCH", "gfx950")
 

@@ -93,6 +93,95 @@ struct HostBuf {
     }
 };
 
+// The Morton-sorted view of a j set (see jprep.hip) and the scratch of its
+// preparation.
+struct SortedView {
+    DevBuf<JRec> jps;
+    DevBuf<JRec32> j32;
+    DevBuf<JTile> tiles;
+    DevBuf<int> jidx;
+    DevBuf<JClus> jcl;
+    DevBuf<PrepClus> pc;
+    DevBuf<unsigned long long> kin, kout;
+    DevBuf<int> vin, vout;
+    DevBuf<char> tmp;
+    size_t tmp_bytes = 0;
+    DevBuf<double> part;
+
+    // cap_j records (a whole number of tiles including one spare tile past
+    // the last one the preparation writes) for ncl clusters.
+    void alloc(hipStream_t st, size_t cap_j, int ncl) {
+        jps.ensure(cap_j);
+        j32.ensure(cap_j);
+        jidx.ensure(cap_j);
+        tiles.ensure(cap_j / kTile + 1);
+        jcl.ensure((size_t)std::max(ncl, 1));
+        pc.ensure((size_t)std::max(ncl, 1));
+        launch_park_view(st, j32.p, (int)cap_j, tiles.p, (int)(cap_j / kTile + 1));
+    }
+    void release() {
+        jps.release();
+        j32.release();
+        tiles.release();
+        jidx.release();
+        jcl.release();
+        pc.release();
+        kin.release();
+        kout.release();
+        vin.release();
+        vout.release();
+        tmp.release();
+        tmp_bytes = 0;
+        part.release();
+    }
+    // Enqueue the preparation of ncl clusters whose table (pc, device) is in
+    // place: they tile [0, ntile_all * kTile) and hold nblk statistics blocks.
+    void prepare(hipStream_t st, const JRec *jp, int ncl, int nblk,
+                 int ntile_all) {
+        if (ntile_all <= 0) return;
+        const size_t n = (size_t)ntile_all * kTile;
+        kin.ensure(n);
+        kout.ensure(n);
+        vin.ensure(n);
+        vout.ensure(n);
+        part.ensure(8 * (size_t)nblk);
+        const size_t need = jprep_tmp_bytes((int)n, ncl);
+        if (need == 0) throw Error("j preparation: sort storage query failed");
+        if (need > tmp_bytes) {
+            tmp.ensure(need);
+            tmp_bytes = need;
+        }
+        JView V;
+        V.jp = jp;
+        V.jps = jps.p;
+        V.j32 = j32.p;
+        V.tiles = tiles.p;
+        V.jidx = jidx.p;
+        V.jcl = jcl.p;
+        PrepBufs B;
+        B.key_in = kin.p;
+        B.key_out = kout.p;
+        B.val_in = vin.p;
+        B.val_out = vout.p;
+        B.tmp = tmp.p;
+        B.tmp_bytes = tmp_bytes;
+        B.part = part.p;
+        HIPCHK(launch_jprep(st, V, pc.p, ncl, nblk, ntile_all, B));
+    }
+    // One cluster covering [0, round_up(nj, kTile)).
+    void prepare_one(hipStream_t st, const JRec *jp, int nj) {
+        const int ntile = (nj + kTile - 1) / kTile;
+        if (ntile == 0) return;
+        PrepClus v;
+        v.j0 = 0;
+        v.nj = nj;
+        v.ntile = ntile;
+        v.blk0 = 0;
+        launch_set_prepclus(st, pc.p, v);
+        prepare(st, jp, 1, prep_stat_blocks(nj), ntile);
+    }
+};
+
 struct State {
     bool devinit = false;
     bool open = false;
@@ -112,6 +201,7 @@ struct State {
     int nj = 0;       // current j count used by regf
     int njp = 0;      // j slots covered by the predictor state
     DevBuf<JRec> jp;
+    SortedView sv;
     DevBuf<JStats> jst;
     DevBuf<unsigned> err;
     HostBuf<unsigned> h_err;
@@ -127,15 +217,23 @@ struct State {
 
     HostBuf<double> h_in;
     DevBuf<double> d_in;
-    DevBuf<double> part;
-    DevBuf<int> cnt;
-    DevBuf<uint16_t> seg;
+    // Regular-force scratch: slice partials, per-i list counts and entries.
     // A call of several chunks runs pass 2 of chunk c on st2 while pass 1 of
-    // chunk c + 1 runs on st into the other scratch set (part_b / cnt_b /
-    // seg_b for odd chunks).
-    DevBuf<double> part_b;
-    DevBuf<int> cnt_b;
-    DevBuf<uint16_t> seg_b;
+    // chunk c + 1 runs on st into the other scratch set (index 1 for odd
+    // chunks).  The counts are zero between launches (zeroed when allocated,
+    // reset by pass 2); nbc_dirty marks a set whose pass 1 was enqueued
+    // without its pass 2.
+    DevBuf<double> part[2];
+    DevBuf<int> nbc[2];
+    DevBuf<int> nbl[2];
+    bool nbc_dirty[2] = {false, false};
+    // Morton order of the i of a call (large calls): keys, permutation, sort
+    // storage and the i data gathered into that order.
+    DevBuf<unsigned> ikey_in, ikey_out;
+    DevBuf<int> ival_in, iperm;
+    DevBuf<char> itmp;
+    size_t itmp_bytes = 0;
+    DevBuf<double> isorted;   // xs [ni][3], vs [ni][3], hs [ni], ds [ni]
     hipStream_t st2 = nullptr;
     hipEvent_t ev_p1[2] = {nullptr, nullptr}, ev_p2[2] = {nullptr, nullptr};
     hipEvent_t ev_p2end = nullptr;
@@ -179,6 +277,7 @@ struct State {
 
 State S;
 bool g_abort = true;
+
 bool g_abort_set = false;
 std::string g_last_error;
 std::mutex g_error_mu;   // entry points may fail on several threads at once
@@ -337,27 +436,32 @@ int resident_waves(int m_flag, int group, int prec) {
     return r;
 }
 
-struct Decomp {
-    int nslice;
-    int cap;
-};
-
 // j slices per i-wavefront: enough waves to fill the resident capacity
-// `rounds` times; slices hold whole tiles, at most 1024 tiles (uint16 local
-// neighbour indices).
-Decomp decompose(int ni_c, int ntile, int nnbmax, int m_flag, int group,
-                 int prec) {
-    Decomp d;
+// `rounds` times; slices hold whole tiles, at most 1024 tiles (the FP64
+// queue holds slice-local 16-bit positions).
+int decompose(int ni_c, int ntile, int m_flag, int group, int prec) {
     const int n_iw = cdiv(ni_c, 64);
     const double target = resident_waves(m_flag, group, prec) * pick_rounds(ni_c);
     int ns = (int)std::lround(target / n_iw);
     ns = std::max(ns, 1);
     ns = std::min(ns, std::max(ntile, 1));
     ns = std::max(ns, cdiv(ntile, 1024));
-    d.nslice = ns;
-    const int max_tiles = cdiv(ntile, ns);
-    d.cap = std::max(1, std::min(max_tiles * kTile, nnbmax + 1));
-    return d;
+    return ns;
+}
+
+// A count buffer of at least n ints, zero where newly allocated.
+void ensure_counts(DevBuf<int> &b, size_t n) {
+    if (n <= b.n) return;
+    b.ensure(n);
+    launch_zero_int(S.st, b.p, (int)b.n);
+}
+
+// Calls of at least this many i are run in Morton order of the i
+// (GPUNB_ISORT=0 switches it off, GPUNB_ISORT=1 forces it on).
+bool pick_isort(int ni) {
+    const char *e = getenv("GPUNB_ISORT");
+    if (e && *e) return atoi(e) != 0;
+    return ni >= 8192;
 }
 
 // i-particles per pass-1/pass-2 launch pair (GPUNB_CHUNK overrides).
@@ -377,11 +481,15 @@ int timing_event() {
     return S.ev_used++;
 }
 
-void ensure_regf_scratch(int ni_c, const Decomp &d, int set) {
-    const size_t pairs = (size_t)round_up(ni_c, 64) * d.nslice;
-    (set ? S.part_b : S.part).ensure(pairs * 8);
-    (set ? S.cnt_b : S.cnt).ensure(pairs);
-    (set ? S.seg_b : S.seg).ensure(pairs * d.cap);
+void ensure_regf_scratch(int ni_c, int nslice, int cap, int set) {
+    const size_t nip = (size_t)round_up(ni_c, 64);
+    S.part[set].ensure(nip * nslice * 8);
+    ensure_counts(S.nbc[set], nip);
+    S.nbl[set].ensure(nip * cap);
+    if (S.nbc_dirty[set]) {
+        launch_zero_int(S.st, S.nbc[set].p, (int)S.nbc[set].n);
+        S.nbc_dirty[set] = false;
+    }
 }
 
 // Enqueue the regular-force kernels for device-resident i data.
@@ -398,12 +506,45 @@ void enqueue_regf(int ni, const double *xi, const double *vi, const double *h2,
     const int ntile = cdiv(S.nj, kTile);
     const int chunk = pick_chunk();
     const int nch = cdiv(ni, chunk);
+    const int *perm = nullptr;
+    if (S.nj > 0 && pick_isort(ni)) {
+        const size_t n = (size_t)ni;
+        S.ikey_in.ensure(n);
+        S.ikey_out.ensure(n);
+        S.ival_in.ensure(n);
+        S.iperm.ensure(n);
+        S.isorted.ensure(8 * n);
+        const size_t need = isort_tmp_bytes(ni);
+        if (need == 0) throw Error("i sort: storage query failed");
+        if (need > S.itmp_bytes) {
+            S.itmp.ensure(need);
+            S.itmp_bytes = need;
+        }
+        ISortBufs B;
+        B.key_in = S.ikey_in.p;
+        B.key_out = S.ikey_out.p;
+        B.val_in = S.ival_in.p;
+        B.perm = S.iperm.p;
+        B.tmp = S.itmp.p;
+        B.tmp_bytes = S.itmp_bytes;
+        B.xs = S.isorted.p;
+        B.vs = B.xs + 3 * n;
+        B.hs = B.vs + 3 * n;
+        B.ds = B.hs + n;
+        HIPCHK(launch_isort(S.st, S.sv.jcl.p, xi, vi, h2, dtr, ni, B));
+        xi = B.xs;
+        vi = B.vs;
+        h2 = B.hs;
+        dtr = B.ds;
+        perm = B.perm;
+    }
     std::vector<ClusterDesc> nd((size_t)nch);
     for (int c = 0; c < nch; ++c) {
         const int c0 = c * chunk;
         const int nc = std::min(chunk, ni - c0);
-        Decomp d = decompose(nc, ntile, nnbmax, m_flag, group, prec);
-        ensure_regf_scratch(nc, d, c & 1);
+        const int ns = ntile == 0 ? 1 : decompose(nc, ntile, m_flag, group, prec);
+        const int cap = std::max(nnbmax, 1);
+        ensure_regf_scratch(nc, ns, cap, c & 1);
         ClusterDesc &D = nd[c];
         memset(&D, 0, sizeof D);
         D.nj = S.nj;
@@ -411,8 +552,8 @@ void enqueue_regf(int ni, const double *xi, const double *vi, const double *h2,
         D.i0 = c0;
         D.ni = nc;
         D.n_iw = cdiv(nc, 64);
-        D.nslice = ntile == 0 ? 1 : d.nslice;
-        D.cap = ntile == 0 ? 1 : d.cap;
+        D.nslice = ns;
+        D.cap = cap;
         D.ipb = pass2_ipb(nc);
     }
     const size_t had = S.d_desc.n;
@@ -434,16 +575,21 @@ void enqueue_regf(int ni, const double *xi, const double *vi, const double *h2,
         const ClusterDesc &D = nd[c];
         RegfLaunch L;
         memset(&L, 0, sizeof L);
-        L.jp = S.jp.p;
+        L.jps = S.sv.jps.p;
+        L.j32 = S.sv.j32.p;
+        L.tiles = S.sv.tiles.p;
+        L.jidx = S.sv.jidx.p;
+        L.jcl = S.sv.jcl.p;
         L.jst = S.jst.p;
         L.xi = xi;
         L.vi = vi;
         L.h2 = h2;
         L.dtr = dtr;
         const int b = c & 1;
-        L.part = b ? S.part_b.p : S.part.p;
-        L.cnt = b ? S.cnt_b.p : S.cnt.p;
-        L.seg = b ? S.seg_b.p : S.seg.p;
+        L.part = S.part[b].p;
+        L.nbc = S.nbc[b].p;
+        L.nbl = S.nbl[b].p;
+        L.perm = perm;
         L.acc = acc;
         L.jrk = jrk;
         L.pot = pot;
@@ -470,6 +616,7 @@ void enqueue_regf(int ni, const double *xi, const double *vi, const double *h2,
             e0 = timing_event();
             HIPCHK(hipEventRecord(S.ev_pool[e0], S.st));
         }
+        S.nbc_dirty[b] = true;
         launch_regf_pass1(S.st, L);
         if (S.timing) {
             e1 = timing_event();
@@ -487,6 +634,7 @@ void enqueue_regf(int ni, const double *xi, const double *vi, const double *h2,
             launch_regf_pass2(S.st, L);
         }
         HIPCHK(hipGetLastError());
+        S.nbc_dirty[b] = false;
     }
     if (nch > 1) {
         // Everything after this call on st (copies, the next predictor,
@@ -540,6 +688,7 @@ void do_open(int nbmax, int irank) {
     launch_stats_reset(S.st, S.jst.p, nullptr);
     launch_pack_j(S.st, nullptr, nullptr, nullptr, 0, (int)cap, S.jp.p,
                   S.jst.p, S.err.p);
+    S.sv.alloc(S.st, cap, 1);
     HIPCHK(hipStreamSynchronize(S.st));
     S.nbmax = nbmax;
     S.nj = 0;
@@ -559,6 +708,7 @@ void do_close() {
     HIPCHK(hipStreamSynchronize(S.st));
     S.ev_send_live = S.ev_jp_live = S.ev_desc_live = false;
     S.jp.release();
+    S.sv.release();
     S.d_send.release();
     S.h_send.release();
     S.x0.release();
@@ -573,12 +723,19 @@ void do_close() {
     S.d_addr.release();
     S.h_in.release();
     S.d_in.release();
-    S.part.release();
-    S.cnt.release();
-    S.seg.release();
-    S.part_b.release();
-    S.cnt_b.release();
-    S.seg_b.release();
+    for (int b = 0; b < 2; ++b) {
+        S.part[b].release();
+        S.nbc[b].release();
+        S.nbl[b].release();
+        S.nbc_dirty[b] = false;
+    }
+    S.ikey_in.release();
+    S.ikey_out.release();
+    S.ival_in.release();
+    S.iperm.release();
+    S.itmp.release();
+    S.itmp_bytes = 0;
+    S.isorted.release();
     S.d_res.release();
     S.h_res.release();
     S.rc_nlist.release();
@@ -621,6 +778,7 @@ void do_send(int nj, const double *m, const double (*x)[3],
     launch_stats_reset(S.st, S.jst.p, nullptr);
     launch_pack_j(S.st, S.d_send.p, S.d_send.p + nj, S.d_send.p + 4 * (size_t)nj,
                   nj, pad, S.jp.p, S.jst.p, S.err.p);
+    S.sv.prepare_one(S.st, S.jp.p, nj);
     HIPCHK(hipGetLastError());
     S.nj = nj;
     S.time_send += wtime() - t0;
@@ -633,6 +791,7 @@ void do_regf(int ni, const double *h2, const double *dtr, const double (*xi)[3],
     require_open("gpunb_regf_");
     REQUIRE(ni >= 0, "gpunb_regf_: ni must be >= 0");
     REQUIRE(nnbmax >= 0, "gpunb_regf_: nnbmax must be >= 0");
+    REQUIRE(nnbmax <= kMaxNnbmax, "gpunb_regf_: nnbmax above 4000 is not supported (a list row is sorted in LDS; the reference LMAX is at most 600)");
     REQUIRE(lmax > nnbmax, "gpunb_regf_: lmax must exceed nnbmax");
     if (ni == 0) return;
     double t0 = wtime();
@@ -825,6 +984,7 @@ void do_predict(double t) {
     launch_stats_reset(S.st, S.jst.p, nullptr);
     launch_predict(S.st, t, S.njp, pad, S.x0.p, S.v0.p, S.f.p, S.fd.p, S.t0.p,
                    S.mass.p, S.jp.p, S.jst.p);
+    S.sv.prepare_one(S.st, S.jp.p, S.njp);
     HIPCHK(hipGetLastError());
     S.nj = S.njp;
 }
@@ -858,14 +1018,16 @@ struct Ens {
     std::vector<int> capj, j0, nj;
     int jcap = 0;
     DevBuf<JRec> jp;
+    SortedView sv;
     DevBuf<JStats> jst;
     HostBuf<double> h_send;
     DevBuf<double> d_send;
     HostBuf<ClusterDesc> h_desc;
     DevBuf<ClusterDesc> d_desc;
     DevBuf<double> part;
-    DevBuf<int> cnt;
-    DevBuf<uint16_t> seg;
+    DevBuf<int> nbc;   // zero between launches, as the engine's
+    DevBuf<int> nbl;
+    bool nbc_dirty = false;
     std::vector<int> bucket_tiles = std::vector<int>(24, 0);
     bool graph_on = false;
     hipGraph_t graph = nullptr;
@@ -889,14 +1051,16 @@ void ens_close() {
     HIPCHK(hipStreamSynchronize(S.st));
     ens_graph_drop();
     E.jp.release();
+    E.sv.release();
     E.jst.release();
     E.h_send.release();
     E.d_send.release();
     E.h_desc.release();
     E.d_desc.release();
     E.part.release();
-    E.cnt.release();
-    E.seg.release();
+    E.nbc.release();
+    E.nbl.release();
+    E.nbc_dirty = false;
     E.open = false;
     E.ncl = 0;
 }
@@ -923,6 +1087,7 @@ void ens_open(int ncl, const int *nbmax) {
     launch_stats_reset(S.st, E.jst.p, nullptr);
     launch_pack_j(S.st, nullptr, nullptr, nullptr, 0, E.jcap, E.jp.p, E.jst.p,
                   S.err.p);
+    E.sv.alloc(S.st, (size_t)E.jcap, ncl);
     HIPCHK(hipStreamSynchronize(S.st));
     E.open = true;
 }
@@ -956,6 +1121,23 @@ void ens_send(int ncl, const int *nj, const double *m, const double (*x)[3],
         E.nj[c] = nj[c];
         o += nj[c];
     }
+    // Sorted view of every cluster: each one's whole capacity range (its
+    // spare records are padding and sort last), so the ranges tile the
+    // array.
+    std::vector<PrepClus> pcs((size_t)ncl);
+    int blk = 0;
+    for (int c = 0; c < ncl; ++c) {
+        pcs[c].j0 = E.j0[c];
+        pcs[c].nj = nj[c];
+        pcs[c].ntile = (c + 1 < ncl ? E.j0[c + 1] : E.jcap - kTile) / kTile -
+                       E.j0[c] / kTile;
+        pcs[c].blk0 = blk;
+        blk += prep_stat_blocks(nj[c]);
+    }
+    HIPCHK(hipMemcpyAsync(E.sv.pc.p, pcs.data(), ncl * sizeof(PrepClus),
+                          hipMemcpyHostToDevice, S.st));
+    E.sv.prepare(S.st, E.jp.p, ncl, blk, (E.jcap - kTile) / kTile);
+    HIPCHK(hipStreamSynchronize(S.st));   // pcs leaves scope
     HIPCHK(hipGetLastError());
 }
 
@@ -990,6 +1172,7 @@ void ens_regf_dev(int ncl, const int *ni, const double *h2, const double *dtr,
     REQUIRE(E.open, "gpunb_ens_regf_dev: ensemble is not open");
     REQUIRE(ncl == E.ncl, "gpunb_ens_regf_dev: ncl differs from gpunb_ens_open");
     REQUIRE(nnbmax >= 0, "gpunb_ens_regf_dev: nnbmax must be >= 0");
+    REQUIRE(nnbmax <= kMaxNnbmax, "gpunb_ens_regf_dev: nnbmax above 4000 is not supported (a list row is sorted in LDS; the reference LMAX is at most 600)");
     set_device();
     const int group = pick_group();
     const int prec = pick_prec();
@@ -1015,7 +1198,7 @@ void ens_regf_dev(int ncl, const int *ni, const double *h2, const double *dtr,
     HIPCHK(hipStreamSynchronize(S.st));   // the table may be in use
     E.h_desc.ensure((size_t)ncl);
     E.d_desc.ensure((size_t)ncl);
-    long long task = 0, pairs = 0, segel = 0, blk = 0, i0 = 0, work = 0;
+    long long task = 0, pairs = 0, blk = 0, i0 = 0, work = 0;
     for (int c = 0; c < ncl; ++c) {
         REQUIRE(ni[c] >= 0, "gpunb_ens_regf_dev: ni must be >= 0");
         ClusterDesc &D = E.h_desc.p[c];
@@ -1031,38 +1214,46 @@ void ens_regf_dev(int ncl, const int *ni, const double *h2, const double *dtr,
         D.n_iw = cdiv(ni[c], 64);
         D.nslice = std::max(1, std::max(cdiv(ntile, Lc), cdiv(ntile, 1024)));
         D.nslice = std::min(D.nslice, std::max(ntile, 1));
-        D.cap = std::max(1, std::min(cdiv(ntile, D.nslice) * kTile, nnbmax + 1));
+        D.cap = std::max(nnbmax, 1);
         D.ipb = pass2_ipb(ni[c]);
+        D.jc = c;
+        D.ibase = (int)i0;
         D.task0 = task;
         D.pbase = pairs;
-        D.sbase = segel;
+        D.sbase = i0 * D.cap;
         D.blk0 = blk;
         task += (long long)D.n_iw * D.nslice;
-        const long long pr = (long long)round_up(std::max(ni[c], 1), 64) * D.nslice;
-        pairs += pr;
-        segel += pr * D.cap;
+        pairs += (long long)round_up(std::max(ni[c], 1), 64) * D.nslice;
         blk += cdiv(ni[c], D.ipb);
         i0 += ni[c];
         work += (long long)ni[c] * E.nj[c];
     }
     E.last_pairs = work;
     E.part.ensure((size_t)pairs * 8);
-    E.cnt.ensure((size_t)pairs);
-    E.seg.ensure((size_t)std::max<long long>(segel, 1));
+    ensure_counts(E.nbc, (size_t)std::max<long long>(i0, 1));
+    E.nbl.ensure((size_t)std::max<long long>(i0 * std::max(nnbmax, 1), 1));
+    if (E.nbc_dirty) {
+        launch_zero_int(S.st, E.nbc.p, (int)E.nbc.n);
+        E.nbc_dirty = false;
+    }
     HIPCHK(hipMemcpyAsync(E.d_desc.p, E.h_desc.p, ncl * sizeof(ClusterDesc),
                           hipMemcpyHostToDevice, S.st));
     HIPCHK(hipStreamSynchronize(S.st));
     RegfLaunch R;
     memset(&R, 0, sizeof R);
-    R.jp = E.jp.p;
+    R.jps = E.sv.jps.p;
+    R.j32 = E.sv.j32.p;
+    R.tiles = E.sv.tiles.p;
+    R.jidx = E.sv.jidx.p;
+    R.jcl = E.sv.jcl.p;
     R.jst = E.jst.p;
     R.xi = xi;
     R.vi = vi;
     R.h2 = h2;
     R.dtr = dtr;
     R.part = E.part.p;
-    R.cnt = E.cnt.p;
-    R.seg = E.seg.p;
+    R.nbc = E.nbc.p;
+    R.nbl = E.nbl.p;
     R.acc = acc;
     R.jrk = jrk;
     R.pot = pot;
@@ -1082,9 +1273,11 @@ void ens_regf_dev(int ncl, const int *ni, const double *h2, const double *dtr,
     R.ntask = task;
     R.nblk2 = blk;
     auto enqueue = [&]() {
+        E.nbc_dirty = true;
         launch_zero_int(S.st, total, 1);
         launch_regf_pass1(S.st, R);
         launch_regf_pass2(S.st, R);
+        E.nbc_dirty = false;
     };
     if (!E.graph_on) {
         enqueue();
@@ -1199,6 +1392,7 @@ GPUNB_EXPORT int gpunb_regf_dev(int ni, const double *d_h2, const double *d_dtr,
     return guarded("gpunb_regf_dev", [&] {
         require_open("gpunb_regf_dev");
         REQUIRE(ni >= 0 && nnbmax >= 0, "gpunb_regf_dev: bad sizes");
+        REQUIRE(nnbmax <= kMaxNnbmax, "gpunb_regf_dev: nnbmax above 4000 is not supported (a list row is sorted in LDS; the reference LMAX is at most 600)");
         set_device();
         launch_zero_int(S.st, d_total, 1);
         if (ni == 0) return;
@@ -1273,7 +1467,7 @@ static_assert(sizeof(gpunb_energy_par) == 4 * 4 + 8, "gpunb_energy_par layout");
 
 // Layouts shared with ctypes (gpunb.RegcorPar / RegcorState) and the
 // Fortran bind(C) types of gpunb_iface.f90.
-static_assert(sizeof(gpunb_regcor_par) == 14 * 8 + 8 * 4,
+static_assert(sizeof(gpunb_regcor_par) == 14 * 8 + 12 * 4,
               "gpunb_regcor_par layout");
 static_assert(sizeof(gpunb_regcor_state) == 22 * sizeof(void *),
               "gpunb_regcor_state layout");
@@ -1300,6 +1494,10 @@ GPUNB_EXPORT int gpunb_regcor_dev(const gpunb_regcor_par *par,
                 "(send or jpset and predict them first)");
         REQUIRE(P.nnbmax >= 0 && P.lmax >= P.nnbmax + 3,
                 "gpunb_regcor_dev: need lmax >= nnbmax + 3");
+        REQUIRE(P.kz13 == 0 && P.kz14 == 0 && P.kz37 == 0 && P.kz38 == 0,
+                "gpunb_regcor_dev: KZ(13), KZ(14), KZ(37) and KZ(38) must be 0 "
+                "(cloud forces, external force, CHECKL and FPCORR of "
+                "REGCOR_GPU are not on the device)");
         REQUIRE((P.nb_flag == 0 || P.nb_flag == 1) &&
                     (P.m_flag == 0 || P.m_flag == 1),
                 "gpunb_regcor_dev: nb_flag and m_flag must be 0 or 1");

@@ -32,6 +32,72 @@ struct alignas(32) PRec {
 };
 static_assert(sizeof(PRec) == 32, "PRec must be 32 bytes");
 
+// The j set in Morton order, as the FP32 pair loop reads it.  The sorted
+// positions are grouped in tiles of kTile spatially close j; every j is held
+// as an FP32 offset from its tile's FP64 centre, so a pair difference is
+// (C_t - x_i) + o_j: the first term once per (i, tile) in FP64 rounded to
+// FP32, the second a plain FP32 add per pair.  32 bytes: one s_load_dwordx8.
+struct alignas(32) JRec32 {
+    float ox, oy, oz;    // position - tile centre (FP32 rounded)
+    float m;             // mass (FP32)
+    float ovx, ovy, ovz; // velocity - tile velocity centre
+    float w;             // near-band weight max(1, x^(2/3), 1.4 x / K), x = m / <m>
+};
+static_assert(sizeof(JRec32) == 32, "JRec32 must be 32 bytes");
+
+// One tile of the sorted j set: FP64 centres (midpoints of the bounding
+// boxes of the tile's positions and velocities), upper bounds on |o| and
+// |ov| of its FP32 records, and the largest weight w.  64 bytes.
+struct alignas(64) JTile {
+    double cx, cy, cz;
+    double cu, cv, cw;
+    float e;     // >= max |(ox, oy, oz)| over the tile's real j
+    float ev;    // >= max |(ovx, ovy, ovz)|
+    float wmax;  // >= max w
+    float eb;    // (kTileBand e)^2: pairs closer than kTileBand e to the
+                 // tile centre's offset scale are evaluated in FP64
+};
+static_assert(sizeof(JTile) == 64, "JTile must be 64 bytes");
+
+// Per-cluster statistics of the j set, reduced in a fixed order (so every
+// run derives the same thresholds from them): bounding box of the positions
+// (the Morton quantisation box) and the mean mass.
+struct JClus {
+    double lo[3];
+    double hi[3];
+    double mbar;
+    double pad;
+};
+
+// Near band of the FP32 pair loop: pairs with min(r2, q) < K_BAND * h2 * w_j
+// (times <m> with mass-weighted lists) are evaluated in FP64.
+constexpr float kBand = 12.0f;
+// Tile floor of the near band: the tile-relative FP32 difference of a pair
+// carries up to 2u (|dx| + e_t) per component, so pairs with |dx| below
+// kTileBand e_t are also evaluated in FP64 (this matters only where the
+// neighbour sphere is small against the tile, e.g. h2 = 0).
+constexpr float kTileBand = 0.5f;
+
+// Cluster of a j preparation: j range [j0, j0 + ntile * kTile) of the
+// original and sorted arrays (tile aligned), nj real j at its start, and its
+// first statistics block.
+struct PrepClus {
+    int j0;
+    int nj;
+    int ntile;
+    int blk0;
+};
+
+// Buffers of the sorted j view (one per engine; the ensemble has its own).
+struct JView {
+    const JRec *jp;     // original order (input of the preparation)
+    JRec *jps;          // FP64 records in Morton order
+    JRec32 *j32;        // FP32 tile-relative records in Morton order
+    JTile *tiles;
+    int *jidx;          // sorted position -> original j (local to the cluster)
+    JClus *jcl;         // per cluster
+};
+
 // Statistics of the current j set, computed on the device at send/predict:
 // max |v_j| and min positive m_j, stored as the bit patterns of the FP64
 // values (non-negative doubles order like their bit patterns, so integer
@@ -62,8 +128,8 @@ constexpr int kTile = 64;
 // grid, each block finding its cluster by binary search over task0 / blk0.
 struct ClusterDesc {
     long long task0;   // first pass-1 task (wavefront) of the cluster
-    long long pbase;   // first (i, slice) pair of the cluster in part / cnt
-    long long sbase;   // first element of the cluster's segments in seg
+    long long pbase;   // first (i, slice) pair of the cluster in part
+    long long sbase;   // first element of the cluster's list entries in nbl
     long long blk0;    // first pass-2 workgroup of the cluster
     int j0;            // first JRec of the cluster (multiple of kTile)
     int nj;            // real j count
@@ -72,14 +138,20 @@ struct ClusterDesc {
     int ni;
     int n_iw;          // i wavefronts: ceil(ni / 64)
     int nslice;        // j slices per i wavefront
-    int cap;           // neighbour segment capacity per (i, slice)
+    int cap;           // list entries stored per i (max(nnbmax, 1))
     int ipb;           // pass-2 i per workgroup
-    int pad;
+    int jc;            // index of the cluster's JClus
+    int ibase;         // first i of the cluster in the list counts nbc
 };
 
 struct RegfLaunch {
-    // j side
-    const JRec *jp;
+    // j side: the sorted view (pass 1 reads the sorted records, pass 2 maps
+    // sorted positions back to original indices)
+    const JRec *jps;
+    const JRec32 *j32;
+    const JTile *tiles;
+    const int *jidx;
+    const JClus *jcl;
     int nj;          // real j count
     int ntile;       // padded j count / kTile
     const JStats *jst;
@@ -88,11 +160,13 @@ struct RegfLaunch {
     int ni;
     // decomposition
     int nslice;      // j slices per i-wave
-    int cap;         // neighbour segment capacity per (i, slice)
     // pass-1 scratch
-    double *part;    // [ni_pad][nslice][8]
-    int *cnt;        // [ni_pad][nslice]
-    uint16_t *seg;   // [ni_pad][nslice][cap]
+    double *part;    // [ni_pad][nslice][8] slice partials
+    int *nbc;        // [ni] list counts: zero before pass 1, reset by pass 2
+    int *nbl;        // [ni][cap] list entries (sorted positions)
+    // i in sorted order: output row of the i at (call-wide) position p is
+    // perm[p]; NULL when the i arrays are in the caller's order
+    const int *perm;
     // outputs
     double *acc, *jrk, *pot;
     int *nnb, *nnid, *list, *off, *total;
@@ -122,9 +196,52 @@ void launch_jpset(hipStream_t st, int n, const int *addr, const double *src,
                   double *x0, double *v0, double *f, double *fd, double *mass,
                   double *t0, unsigned *err);
 void launch_jget(hipStream_t st, int n, const JRec *jp, double *x, double *v);
+
+// j preparation (jprep.hip): per-cluster statistics, Morton sort and the
+// records of the sorted view, for ncl clusters whose ranges (device table pc)
+// tile [0, ntile_all * kTile).  nblk = sum over clusters of
+// prep_stat_blocks(nj).
+constexpr int kStatChunk = 2048;   // j per statistics block
+inline int prep_stat_blocks(int nj) {
+    return nj > 0 ? (nj + kStatChunk - 1) / kStatChunk : 1;
+}
+struct PrepBufs {
+    unsigned long long *key_in, *key_out;
+    int *val_in, *val_out;
+    void *tmp;
+    size_t tmp_bytes;
+    double *part;      // 8 doubles per statistics block
+};
+// rocprim temporary storage for sorting n keys of ncl clusters.
+size_t jprep_tmp_bytes(int n, int ncl);
+hipError_t launch_jprep(hipStream_t st, const JView &V, const PrepClus *pc,
+                        int ncl, int nblk, int ntile_all, const PrepBufs &B);
+// Morton order of the i of one call (jprep.hip): perm[p] = the original i
+// at sorted position p, and the i data gathered into that order.
+struct ISortBufs {
+    unsigned *key_in, *key_out;
+    int *val_in, *perm;
+    void *tmp;
+    size_t tmp_bytes;
+    double *xs, *vs, *hs, *ds;   // [ni][3], [ni][3], [ni], [ni]
+};
+size_t isort_tmp_bytes(int n);
+hipError_t launch_isort(hipStream_t st, const JClus *jcl, const double *xi,
+                        const double *vi, const double *h2, const double *dtr,
+                        int ni, const ISortBufs &B);
+// *pc = v, in stream order.
+void launch_set_prepclus(hipStream_t st, PrepClus *pc, const PrepClus &v);
+// Padding records in j32[0, n) and empty tiles in tiles[0, ntile).
+void launch_park_view(hipStream_t st, JRec32 *j32, int n, JTile *tiles,
+                      int ntile);
 void launch_regf_pass1(hipStream_t st, const RegfLaunch &L);
 void launch_regf_pass2(hipStream_t st, const RegfLaunch &L);
 int pass2_ipb(int ni);   // pass-2 i per workgroup for a cluster of ni
+// Entries of a pass-2 wavefront's LDS list row (>= nnbmax + 1); the row of
+// the four wavefronts of a workgroup must fit the 64 KiB of LDS a
+// workgroup can address, hence the largest supported nnbmax.
+int pass2_sortcap(int nnbmax);
+constexpr int kMaxNnbmax = 4000;
 void launch_zero_int(hipStream_t st, int *p, int n);
 // bytes (a multiple of 16, both pointers 16-byte aligned) from device memory
 // to a mapped pinned host buffer, and *err to *herr (host-mapped).

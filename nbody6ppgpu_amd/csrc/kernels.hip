@@ -312,32 +312,75 @@ __global__ void k_zero_int(int *p, int n) {
 
 // ---------------------------------------------------------------------------
 // Regular force, pass 1: one wavefront = 64 i-particles (one per lane) x one
-// contiguous j slice.  j records are wave-uniform, so they arrive through the
-// scalar cache as SGPR operands: no LDS and no VGPRs are spent on them.
+// contiguous slice of the Morton-sorted j set.  j records are wave-uniform,
+// so they arrive through the scalar cache as SGPR operands.
 //
-// Per pair (fast path): FP64 differences dx = x_j - x_i, dv = v_j - v_i
-// (exactly the oracle's), rounded to FP32; FP32 r2, dx.dv, the screen value
+// Pair differences.  Every j is stored as an FP32 offset o_j from the FP64
+// centre C_t of its tile (64 spatially close j), so per (lane, tile) the
+// FP64 difference C_t - x_i is formed once and rounded to FP32 (D), and per
+// pair dx = D + o_j is one FP32 add per component (velocities alike).  With
+// |o_j| <= e_t (the tile's offset bound) the rounding error is, per
+// component, at most 2u (|dx_c| + |o_c|) (u = 2^-24): D carries u |C - x_i|
+// <= u (|dx_c| + |o_c|), o_j u |o_c|, the add u |dx_c|.
+//
+// Screen.  A pair is flagged when the FP32 screen value s = min(r2, q),
 // q = r2 + 2 dtr dx.dv (a lower bound of the predicted distance r2p = q +
-// dtr^2 |dv|^2), one v_rsq_f32 and the force sums.  The neighbour test first
-// compares the FP32 min(r2, q) with a per-lane threshold that is provably
-// above the FP64 threshold by more than the FP32 evaluation error; only
-// lanes below it take the FP64 path, which evaluates the oracle's predicate
-// bit for bit.  Pairs that are no neighbours therefore never pay for FP64
-// arithmetic beyond the six differences, and the lists are exactly the FP64
-// lists.  The j records of the next group are loaded while the current
-// group is evaluated.
+// dtr^2 |dv|^2, so min(r2, q) <= min(r2, r2p)), lies below T_j = B w_j:
+//   B   = K h2 (K h2 <m> with mass-weighted lists), K = kBand, per lane
+//   w_j = max(1, x^(2/3), 1.4 x / K), x = m_j / <m>, per j,
+// and B is raised per tile to (e_t / 2)^2 (e_t: the tile's offset bound), so
+// pairs whose tile-relative difference would carry a large relative error
+// (|dx| not well above e_t) are among them.
+// Flagged pairs are the near band: the list candidates and the pairs close
+// enough (relative to the neighbour sphere, and more so for heavy j whose
+// terms are large) that their FP32 rounding would dominate the jerk error.
+// They leave the FP32 sums and go to the lane's queue in LDS; at the end of
+// a tile after which a lane's queue might overflow, and after the last
+// tile, the wave evaluates every queued pair in FP64 (lanes in step, each
+// on its own queue): the oracle's neighbour predicate bit for bit (a
+// neighbour is appended to the lane's segment in j order), otherwise the
+// FP64 force terms (1/r by v_rsq_f64 and one Newton step) into the FP64 sums.
 //
-// The FP32 partial sums are folded into FP64 accumulators every kTile pairs.
+// Why no neighbour is lost.  T_j >= 1.4 H_max with H_max = h2 (m_j h2 with
+// mass-weighted lists), so the screen misses a neighbour only if s exceeds
+// the FP64 predicate's value by 0.4 H.  Bounding the errors above for a pair
+// with min(r2, r2p) < H, with rho = |dtr| (|v_i| + max|v_j|) / sqrt(H_min):
+//   |s_32 - s| <= u H [23 (1 + rho)^2 + 1]
+//                 + 8u sqrt(H) (1 + 2 rho) (e_t + |dtr| ev_t)
+// (ev_t: the tile's velocity-offset bound), plus the FP64 predicate's own
+// rounding, below 1e-15 (1 + rho)^2 H.  A lane treats a tile as unsafe when
+// this could exceed 0.35 H; then every pair of that tile is flagged (only
+// tiles spanning many neighbour radii are affected, e.g. tiles that straddle
+// two Morton cells).
+//
+// Far tiles.  Per (lane, tile), with d = |D|, R = d - e_t and A = |dtr|
+// (|DV| + ev_t) (|DV| as d), every pair of the tile has |dx_32| >= R and
+// |dtr| |dv_32| <= A up to a few u, so s_32 >= R^2 - 2 A R; when R > A +
+// sqrt(A^2 + B wmax_t) (with a 1e-5 relative margin for the roundings) no
+// pair of the tile can be flagged, and when no lane of the wave is near the
+// tile the wave runs it without the screen.  Both loops evaluate unflagged
+// pairs with the same instructions, so the split does not change results.
+//
+// The FP32 partial sums are folded into FP64 accumulators every tile.
 // ---------------------------------------------------------------------------
+
+// Queued FP64 pairs per lane.  The queue is flushed at tile ends only (a
+// flush inside the pair loop would keep its FP64 temporaries live across
+// the loop and cost occupancy), so it holds a whole tile beyond the flush
+// threshold.
+constexpr int kQ = kTile + 16;
 
 template <bool MFLAG, int G>
 __global__ __launch_bounds__(64) void k_regf_pass1(
-    const JRec *__restrict__ jp_all, const ClusterDesc *__restrict__ desc,
-    int ncl, long long ntask, const JStats *__restrict__ jst,
-    const double *__restrict__ xi_all, const double *__restrict__ vi_all,
-    const double *__restrict__ h2_all, const double *__restrict__ dtr_all,
-    double *__restrict__ part_all, int *__restrict__ cnt_all,
-    uint16_t *__restrict__ seg_all, unsigned *__restrict__ err) {
+    const JRec32 *__restrict__ j32_all, const JTile *__restrict__ tile_all,
+    const JRec *__restrict__ jps_all, const JClus *__restrict__ jcl_all,
+    const ClusterDesc *__restrict__ desc, int ncl, long long ntask,
+    const JStats *__restrict__ jst, const double *__restrict__ xi_all,
+    const double *__restrict__ vi_all, const double *__restrict__ h2_all,
+    const double *__restrict__ dtr_all, double *__restrict__ part_all,
+    int *__restrict__ nbc_all, int *__restrict__ nbl_all,
+    unsigned *__restrict__ err) {
+    __shared__ uint16_t qbuf[kQ][64];
     // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
     // contiguous range of tasks so that it streams only its share of the j
     // array through its own L2.  Tasks are slice-major within a cluster.
@@ -347,14 +390,17 @@ __global__ __launch_bounds__(64) void k_regf_pass1(
     const ClusterDesc &D = desc[find_task(desc, ncl, gtask)];
     const int nj = D.nj, ntile = D.ntile, ni = D.ni, n_iw = D.n_iw;
     const int nslice = D.nslice, cap = D.cap;
-    const JRec *__restrict__ jp = jp_all + D.j0;
+    const JRec32 *__restrict__ j32 = j32_all + D.j0;
+    const JTile *__restrict__ tiles = tile_all + D.j0 / kTile;
+    const JRec *__restrict__ jps = jps_all + D.j0;
+    const double mbar = jcl_all[D.jc].mbar;
     const double *__restrict__ xi = xi_all + 3 * (size_t)D.i0;
     const double *__restrict__ vi = vi_all + 3 * (size_t)D.i0;
     const double *__restrict__ h2 = h2_all + D.i0;
     const double *__restrict__ dtr = dtr_all + D.i0;
     double *__restrict__ part = part_all + 8 * D.pbase;
-    int *__restrict__ cnt = cnt_all + D.pbase;
-    uint16_t *__restrict__ seg = seg_all + D.sbase;
+    int *__restrict__ nbc = nbc_all + D.ibase;
+    int *__restrict__ nbl = nbl_all + D.sbase;
     const int task = (int)(gtask - D.task0);
     const int s = task / n_iw;
     const int iw = task - s * n_iw;
@@ -368,8 +414,10 @@ __global__ __launch_bounds__(64) void k_regf_pass1(
 
     double x0 = 0.0, y0 = 0.0, z0 = 0.0, u0 = 0.0, w0 = 0.0, q0 = 0.0;
     double h2d = 0.0, dtrd = 0.0;
-    float dtr2x = 0.0f;   // 2 dtr (FP32)
-    float thr = -1.0f;  // inactive lanes never enter the FP64 path
+    float dtr2x = 0.0f;         // 2 dtr
+    float adtr = 0.0f;          // >= |dtr|
+    float B = -INFINITY;        // band base; inactive lanes never flag
+    float lim = -1.0f;          // e_t + |dtr| ev_t above lim: unsafe tile
     if (act) {
         x0 = xi[3 * (size_t)i + 0];
         y0 = xi[3 * (size_t)i + 1];
@@ -383,58 +431,44 @@ __global__ __launch_bounds__(64) void k_regf_pass1(
         if (chk != chk) atomicOr(err, ERR_NAN_I);
         const float dtr32 = (float)dtrd;
         dtr2x = 2.0f * dtr32;
-        // FP32 screening threshold.  The list predicate is
-        // min(r2, r2p) < H with r2p = |dx + dtr dv|^2 = q + dtr^2 |dv|^2 and
-        // q = r2 + 2 dtr dx.dv; since dtr^2 |dv|^2 >= 0, min(r2, q) <= min(r2,
-        // r2p), so the screen value min(r2, q) flags a superset of the
-        // neighbours (5% more pairs on a King W0=6 model) without |dv|^2.
-        // The fast path evaluates q = fma(2 dtr, dx.dv, r2) from the
-        // FP32-rounded differences; every operand and operation contributes
-        // a relative error of at most u to one of the nonnegative magnitudes
-        // r2, 2|dtr| sum|dx_c dv_c|, so |q_32 - q| <= 7.2u S with
-        // S = sum_c (|dx_c| + |dtr| |dv_c|)^2 <= (|dx| + |dtr| |dv|)^2.  If
-        // r2p < H (H the FP64 threshold: h2, or m_j h2 >= m_min h2), then
-        // |dx| <= sqrt(H) + |dtr| |dv| and, with
-        // rho = |dtr| (|v_i| + max|v_j|) / sqrt(H), S <= H (1 + 2 rho)^2.
-        // r2 itself is within a relative 5.1u.  The threshold allows 20u S
-        // and adds 16u (24u with the mass factor) for the rounding of the
-        // FP64 reference values and of the threshold.
+        adtr = fabsf(dtr32) * (1.0f + 2.0f * kU32);
+        const double u = (double)kU32;
         if (h2d > 0.0) {
+            const double bb = (double)kBand * h2d * (MFLAG ? mbar : 1.0) *
+                              (1.0 + 8.0 * u);
+            B = fmaxf((float)bb * (1.0f + 2.0f * kU32), FLT_MIN);
+            // Unsafe-tile limit from the error bound above.
             const double vmax = bitsd(jst->vmax_bits);
             double hmin = h2d;
             if (MFLAG) {
                 const double mmin = bitsd(jst->mmin_bits);
                 hmin = (mmin < INFINITY) ? h2d * mmin : h2d;
             }
-            const double u = (double)kU32;
-            const double rho =
-                fabs(dtrd) * (speed_up(u0, w0, q0) + vmax) / sqrt(hmin);
-            const double g = 1.0 + 2.0 * rho;
-            const double eps = 2.0 * 10.0 * u * g * g + (MFLAG ? 24.0 : 16.0) * u;
-            const double tt = h2d * (1.0 + eps);
-            float t32 = (float)tt;
-            if ((double)t32 < tt) t32 = nextafterf(t32, INFINITY);
-            thr = fmaxf(t32, FLT_MIN);
+            const double sh = sqrt(hmin);
+            const double rho = fabs(dtrd) * (speed_up(u0, w0, q0) + vmax) / sh;
+            const double g = 1.0 + rho;
+            const double a0 = u * (23.0 * g * g + 1.0) + 1e-15 * g * g;
+            const double a1 = 8.0 * u * (1.0 + 2.0 * rho) / sh;
+            const double slack = 0.35 - a0;
+            lim = slack > 0.0 ? (float)(slack / a1 * (1.0 - 1e-6)) : -1.0f;
         } else {
-            thr = FLT_MIN;  // only r2 == 0 is screened (never a neighbour)
+            // No neighbours (H = 0): only r2 == 0 pairs are screened, so the
+            // self pair never enters the FP32 sums.
+            B = FLT_MIN;
+            lim = INFINITY;
         }
     }
 
     float ax = 0.f, ay = 0.f, az = 0.f, jx = 0.f, jy = 0.f, jz = 0.f, ph = 0.f;
     double Ax = 0.0, Ay = 0.0, Az = 0.0, Jx = 0.0, Jy = 0.0, Jz = 0.0, Ph = 0.0;
-    int c = 0;
-    uint16_t *sg = seg + ((size_t)(act ? i : 0) * nslice + s) * cap;
+    int qn = 0;
 
-    // FP32 force terms of one pair, shared by the screened fast path and the
-    // FP64 resolution of band pairs.
-    auto pair_terms = [&](const JRec &J, float fx, float fy, float fz,
-                          float gx, float gy, float gz, float r2, float rv,
-                          float rinv) {
+    auto terms32 = [&](float m, float fx, float fy, float fz, float gx,
+                       float gy, float gz, float rv, float rinv) {
         const float rinv2 = rinv * rinv;
-        const float mr = J.m32 * rinv;
+        const float mr = m * rinv;
         const float mr3 = mr * rinv2;
         const float al = (-3.0f * rv) * rinv2;
-        (void)r2;
         ph += mr;
         ax = fmaf(mr3, fx, ax);
         ay = fmaf(mr3, fy, ay);
@@ -444,73 +478,183 @@ __global__ __launch_bounds__(64) void k_regf_pass1(
         jz = fmaf(mr3, fmaf(al, fz, gz), jz);
     };
 
-    // Resolution of one screened pair in FP64: the oracle's predicate bit for
-    // bit; a neighbour is appended to the lane's segment, anything else adds
-    // the FP32 force term the screened fast path left out.
-    auto resolve = [&](const JRec &J, int j) {
-        const double dx = J.x - x0, dy = J.y - y0, dz = J.z - z0;
-        const double dvx = J.vx - u0, dvy = J.vy - w0, dvz = J.vz - q0;
-        const double r2d = sq3d(dx, dy, dz);
-        const double pxd = dx + dtrd * dvx;
-        const double pyd = dy + dtrd * dvy;
-        const double pzd = dz + dtrd * dvz;
-        const double r2pd = sq3d(pxd, pyd, pzd);
-        const double mnd = r2d < r2pd ? r2d : r2pd;
-        const double H = MFLAG ? J.m * h2d : h2d;
-        if ((mnd < H) && (j < nj)) {
-            if (c < cap) sg[c] = (uint16_t)(j - jbeg);
-            ++c;
-        } else {
-            const float fx = (float)dx, fy = (float)dy, fz = (float)dz;
-            const float gx = (float)dvx, gy = (float)dvy, gz = (float)dvz;
-            const float r2 = fmaf(fz, fz, fmaf(fy, fy, fx * fx));
-            const float rv = fmaf(fz, gz, fmaf(fy, gy, fx * gx));
-            if (r2 != 0.0f)
-                pair_terms(J, fx, fy, fz, gx, gy, gz, r2, rv,
-                           __builtin_amdgcn_rsqf(r2));
+    // FP64 evaluation of the queued pairs of all lanes, 64 at a time.  The
+    // lanes' queues are taken as one sequence in lane order (lane l's
+    // entries at [excl_l, excl_l + qn_l)); in each batch a lane evaluates
+    // one entry for its owner, whose i data it fetches from the owner lane.
+    // An owner's entries form one contiguous run per batch, so a segmented
+    // suffix sum leaves the run's sums in its first lane, from which the
+    // owner adds them to its FP64 sums: a fixed order, whatever the lanes'
+    // queue lengths.  Neighbours found are appended to the owner's list.
+    auto flush = [&]() {
+        const int incl = wave_incl_scan(qn, lane);
+        const int excl = incl - qn;
+        const int total = __builtin_amdgcn_readfirstlane(__shfl(incl, 63));
+        for (int b0 = 0; b0 < total; b0 += 64) {
+            const int e = b0 + lane;
+            // Owner of entry e: the last lane o with excl_o <= e.
+            int o = 0;
+#pragma unroll
+            for (int step = 32; step >= 1; step >>= 1) {
+                const int c2 = o + step;
+                if (__shfl(excl, c2) <= e) o = c2;
+            }
+            const bool valid = e < total;
+            const int k = e - __shfl(excl, o);
+            const int pl = qbuf[valid ? k : 0][valid ? o : 0];
+            const int p = jbeg + pl;
+            const double ox = __shfl(x0, o), oy = __shfl(y0, o), oz = __shfl(z0, o);
+            const double ou = __shfl(u0, o), ow = __shfl(w0, o), oq = __shfl(q0, o);
+            const double oh2 = __shfl(h2d, o), odt = __shfl(dtrd, o);
+            double r[7] = {0, 0, 0, 0, 0, 0, 0};
+            if (valid) {
+                const JRec J = jps[p];
+                const double dx = J.x - ox, dy = J.y - oy, dz = J.z - oz;
+                const double dvx = J.vx - ou, dvy = J.vy - ow, dvz = J.vz - oq;
+                const double r2d = sq3d(dx, dy, dz);
+                const double pxd = dx + odt * dvx;
+                const double pyd = dy + odt * dvy;
+                const double pzd = dz + odt * dvz;
+                const double r2pd = sq3d(pxd, pyd, pzd);
+                const double mnd = r2d < r2pd ? r2d : r2pd;
+                const double H = MFLAG ? J.m * oh2 : oh2;
+                if ((mnd < H) && (p < nj)) {
+                    // The owner's list: slices append in any order (pass 2
+                    // sorts); entries beyond cap only count (overflow).
+                    const int io = iw * 64 + o;
+                    const int k2 = atomicAdd(&nbc[io], 1);
+                    if (k2 < cap) nbl[(size_t)io * cap + k2] = p;
+                } else if (r2d != 0.0) {
+                    double y = __builtin_amdgcn_rsq(r2d);
+                    y = fma(y, 0.5 * fma(-r2d * y, y, 1.0), y);
+                    const double rinv2 = y * y;
+                    const double mr = J.m * y;
+                    const double mr3 = mr * rinv2;
+                    const double rv = fma(dz, dvz, fma(dy, dvy, dx * dvx));
+                    const double al = -3.0 * rv * rinv2;
+                    r[0] = mr3 * dx;
+                    r[1] = mr3 * dy;
+                    r[2] = mr3 * dz;
+                    r[3] = mr3 * fma(al, dx, dvx);
+                    r[4] = mr3 * fma(al, dy, dvy);
+                    r[5] = mr3 * fma(al, dz, dvz);
+                    r[6] = mr;
+                }
+            }
+            // Segmented suffix sums over runs of equal owner.  Every shuffle
+            // runs with all lanes active (a permute reads 0 from a lane
+            // outside the active mask, so it must not sit under a
+            // lane-dependent condition).
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const int od = __shfl_down(o, d);
+                const bool same = (lane + d < 64) & (od == o);
+#pragma unroll
+                for (int c = 0; c < 7; ++c) {
+                    const double t = __shfl_down(r[c], d);
+                    if (same) r[c] += t;
+                }
+            }
+            // The owner's run in this batch starts at lane max(excl - b0, 0).
+            const bool mine = qn > 0 && incl > b0 && excl < b0 + 64;
+            const int head = excl > b0 ? excl - b0 : 0;
+            double g[7];
+#pragma unroll
+            for (int c = 0; c < 7; ++c) g[c] = __shfl(r[c], head);
+            if (mine) {
+                Ax += g[0];
+                Ay += g[1];
+                Az += g[2];
+                Jx += g[3];
+                Jy += g[4];
+                Jz += g[5];
+                Ph += g[6];
+            }
         }
+        qn = 0;
     };
 
-    // The records of the next group are loaded while the current group is
-    // evaluated, hiding the scalar-load latency inside the wave (5% faster
-    // per launch).  The j array carries 64 spare records past its last tile,
-    // so the look-ahead of the last group stays in bounds.
-    JRec Jn[G];
-#pragma unroll
-    for (int k = 0; k < G; ++k) Jn[k] = jp[tb * kTile + k];
+    // The records of the next group (and the next tile record) are loaded
+    // while the current ones are evaluated; the look-ahead restarts in every
+    // tile, so the prefetched records stay in SGPRs (carried across the
+    // tile's branches the compiler would copy them to VGPRs every group).
+    // The record arrays carry a spare tile past their end, so the last
+    // look-ahead stays in bounds.
+    JTile Tn = tiles[tb];
     for (int t = tb; t < te; ++t) {
+        const JTile T = Tn;
+        Tn = tiles[t + 1];
         const int jt = t * kTile;
-        for (int jg = 0; jg < kTile; jg += G) {
-            JRec J[G];
-            bool fl[G];
-            bool any = false;
+        const float Dx = (float)(T.cx - x0), Dy = (float)(T.cy - y0),
+                    Dz = (float)(T.cz - z0);
+        const float Vx = (float)(T.cu - u0), Vy = (float)(T.cv - w0),
+                    Vz = (float)(T.cw - q0);
+        const bool unsafe = fmaf(adtr, T.ev, T.e) > lim;
+        const float Bt = !act ? B : (unsafe ? INFINITY : fmaxf(B, T.eb));
+        const float d = sqrtf(fmaf(Dz, Dz, fmaf(Dy, Dy, Dx * Dx)));
+        const float dvn = sqrtf(fmaf(Vz, Vz, fmaf(Vy, Vy, Vx * Vx)));
+        const float R = fmaf(d, 1.0f - 1e-5f, -T.e);
+        const float A = adtr * (dvn + T.ev) * (1.0f + 1e-5f);
+        const float Tmax = Bt * T.wmax * (1.0f + 1e-5f);
+        const bool far =
+            !act || (R > fmaf(sqrtf(fmaf(A, A, Tmax)), 1.0f + 1e-5f, A));
+        if (__ballot(!far) == 0) {
+            JRec32 Jn[G];
 #pragma unroll
-            for (int k = 0; k < G; ++k) J[k] = Jn[k];
+            for (int k = 0; k < G; ++k) Jn[k] = j32[jt + k];
+            for (int jg = 0; jg < kTile; jg += G) {
+                JRec32 J[G];
 #pragma unroll
-            for (int k = 0; k < G; ++k) Jn[k] = jp[jt + jg + G + k];
+                for (int k = 0; k < G; ++k) J[k] = Jn[k];
 #pragma unroll
-            for (int k = 0; k < G; ++k) {
-                const double dx = J[k].x - x0;
-                const double dy = J[k].y - y0;
-                const double dz = J[k].z - z0;
-                const float fx = (float)dx, fy = (float)dy, fz = (float)dz;
-                const float gx = (float)(J[k].vx - u0);
-                const float gy = (float)(J[k].vy - w0);
-                const float gz = (float)(J[k].vz - q0);
-                const float r2 = fmaf(fz, fz, fmaf(fy, fy, fx * fx));
-                const float rv = fmaf(fz, gz, fmaf(fy, gy, fx * gx));
-                const float q = fmaf(dtr2x, rv, r2);
-                float tj = thr;
-                if (MFLAG) tj = fmaxf(thr * J[k].m32, FLT_MIN);
-                fl[k] = fminf(r2, q) < tj;
-                any = any || fl[k];
-                const float rinv = fl[k] ? 0.0f : __builtin_amdgcn_rsqf(r2);
-                pair_terms(J[k], fx, fy, fz, gx, gy, gz, r2, rv, rinv);
+                for (int k = 0; k < G; ++k) Jn[k] = j32[jt + jg + G + k];
+#pragma unroll
+                for (int k = 0; k < G; ++k) {
+                    const float fx = Dx + J[k].ox, fy = Dy + J[k].oy,
+                                fz = Dz + J[k].oz;
+                    const float gx = Vx + J[k].ovx, gy = Vy + J[k].ovy,
+                                gz = Vz + J[k].ovz;
+                    const float r2 = fmaf(fz, fz, fmaf(fy, fy, fx * fx));
+                    const float rv = fmaf(fz, gz, fmaf(fy, gy, fx * gx));
+                    terms32(J[k].m, fx, fy, fz, gx, gy, gz, rv,
+                            __builtin_amdgcn_rsqf(r2));
+                }
             }
-            if (any) {
+        } else {
+            JRec32 Jn[G];
 #pragma unroll
-                for (int k = 0; k < G; ++k)
-                    if (fl[k]) resolve(J[k], jt + jg + k);
+            for (int k = 0; k < G; ++k) Jn[k] = j32[jt + k];
+            for (int jg = 0; jg < kTile; jg += G) {
+                JRec32 J[G];
+                bool fl[G];
+                bool any = false;
+#pragma unroll
+                for (int k = 0; k < G; ++k) J[k] = Jn[k];
+#pragma unroll
+                for (int k = 0; k < G; ++k) Jn[k] = j32[jt + jg + G + k];
+#pragma unroll
+                for (int k = 0; k < G; ++k) {
+                    const float fx = Dx + J[k].ox, fy = Dy + J[k].oy,
+                                fz = Dz + J[k].oz;
+                    const float gx = Vx + J[k].ovx, gy = Vy + J[k].ovy,
+                                gz = Vz + J[k].ovz;
+                    const float r2 = fmaf(fz, fz, fmaf(fy, fy, fx * fx));
+                    const float rv = fmaf(fz, gz, fmaf(fy, gy, fx * gx));
+                    const float q = fmaf(dtr2x, rv, r2);
+                    fl[k] = fminf(r2, q) < Bt * J[k].w;
+                    any = any || fl[k];
+                    const float rinv = fl[k] ? 0.0f : __builtin_amdgcn_rsqf(r2);
+                    terms32(J[k].m, fx, fy, fz, gx, gy, gz, rv, rinv);
+                }
+                if (__ballot(any) != 0) {
+#pragma unroll
+                    for (int k = 0; k < G; ++k) {
+                        if (fl[k]) {
+                            qbuf[qn][lane] = (uint16_t)(jt + jg + k - jbeg);
+                            ++qn;
+                        }
+                    }
+                }
             }
         }
         Ax += (double)ax;
@@ -521,6 +665,9 @@ __global__ __launch_bounds__(64) void k_regf_pass1(
         Jz += (double)jz;
         Ph += (double)ph;
         ax = ay = az = jx = jy = jz = ph = 0.f;
+        // A tile adds at most kTile entries per lane: flush when the next
+        // one might not fit, and after the last tile.
+        if (__ballot(qn > kQ - kTile) != 0 || t + 1 == te) flush();
     }
 
     if (act) {
@@ -533,7 +680,6 @@ __global__ __launch_bounds__(64) void k_regf_pass1(
         p[5] = Jz;
         p[6] = Ph;
         p[7] = 0.0;
-        cnt[(size_t)i * nslice + s] = c;
     }
 }
 
@@ -559,8 +705,8 @@ __global__ __launch_bounds__(64) void k_regf_pass1_f64(
     int ncl, long long ntask, const JStats *__restrict__ jst,
     const double *__restrict__ xi_all, const double *__restrict__ vi_all,
     const double *__restrict__ h2_all, const double *__restrict__ dtr_all,
-    double *__restrict__ part_all, int *__restrict__ cnt_all,
-    uint16_t *__restrict__ seg_all, unsigned *__restrict__ err) {
+    double *__restrict__ part_all, int *__restrict__ nbc_all,
+    int *__restrict__ nbl_all, unsigned *__restrict__ err) {
     // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
     // contiguous range of tasks so that it streams only its share of the j
     // array through its own L2.  Tasks are slice-major within a cluster.
@@ -576,8 +722,8 @@ __global__ __launch_bounds__(64) void k_regf_pass1_f64(
     const double *__restrict__ h2 = h2_all + D.i0;
     const double *__restrict__ dtr = dtr_all + D.i0;
     double *__restrict__ part = part_all + 8 * D.pbase;
-    int *__restrict__ cnt = cnt_all + D.pbase;
-    uint16_t *__restrict__ seg = seg_all + D.sbase;
+    int *__restrict__ nbc = nbc_all + D.ibase;
+    int *__restrict__ nbl = nbl_all + D.sbase;
     const int task = (int)(gtask - D.task0);
     const int s = task / n_iw;
     const int iw = task - s * n_iw;
@@ -587,7 +733,6 @@ __global__ __launch_bounds__(64) void k_regf_pass1_f64(
 
     const int tb = (int)(((long long)ntile * s) / nslice);
     const int te = (int)(((long long)ntile * (s + 1)) / nslice);
-    const int jbeg = tb * kTile;
 
     double x0 = 0.0, y0 = 0.0, z0 = 0.0, u0 = 0.0, w0 = 0.0, q0 = 0.0;
     double h2d = 0.0, dtrd = 0.0, dtr2x = 0.0;   // dtr2x = 2 dtr
@@ -631,8 +776,6 @@ __global__ __launch_bounds__(64) void k_regf_pass1_f64(
 
     double ax = 0.0, ay = 0.0, az = 0.0, ph = 0.0;
     double j1x = 0.0, j1y = 0.0, j1z = 0.0, j2x = 0.0, j2y = 0.0, j2z = 0.0;
-    int c = 0;
-    uint16_t *sg = seg + ((size_t)(act ? i : 0) * nslice + s) * cap;
 
     // 1/sqrt(r2): v_rsq_f64 (about FP32-level accuracy) refined by NR Newton
     // steps in FP64 (one step reaches ~1e-14 relative).
@@ -677,8 +820,8 @@ __global__ __launch_bounds__(64) void k_regf_pass1_f64(
         const double mnd = r2d < r2pd ? r2d : r2pd;
         const double H = MFLAG ? J.m * h2d : h2d;
         if ((mnd < H) && (j < nj)) {
-            if (c < cap) sg[c] = (uint16_t)(j - jbeg);
-            ++c;
+            const int k2 = atomicAdd(&nbc[i], 1);
+            if (k2 < cap) nbl[(size_t)i * cap + k2] = j;
         } else if (r2d != 0.0) {
             // Rare (band pairs, r2 < 1e-30): exact FP64 1/sqrt.
             const double rv = fma(dz, dvz, fma(dy, dvy, dx * dvx));
@@ -739,66 +882,69 @@ __global__ __launch_bounds__(64) void k_regf_pass1_f64(
         p[5] = fma(-3.0, j2z, j1z);
         p[6] = ph;
         p[7] = 0.0;
-        cnt[(size_t)i * nslice + s] = c;
     }
 }
 
 // ---------------------------------------------------------------------------
 // Regular force, pass 2: a workgroup of 4 wavefronts serves ipb <= kP2I
 // i-particles (ipb chosen per launch so that small calls still spread over
-// the machine), each wavefront every 4th of them.  Per i: the slice partials are summed
-// in a fixed order (deterministic FP64) and the forces written; then one
-// thread scans the block's list lengths and claims the block's list space
-// with a single atomic; then every i's per-slice neighbour segments are
-// concatenated in slice order (= ascending j) and its nearest neighbour found
-// (smallest FP64 r2 > 0, ties to the smaller j).
+// the machine), each wavefront every 4th of them.  Per i: the slice partials
+// are summed in a fixed order (deterministic FP64) and the forces written,
+// and the list count read (and reset for the next launch on this scratch);
+// then one thread scans the block's list lengths and claims the block's list
+// space with a single atomic; then every i's list entries (sorted positions,
+// appended by the slices in any order) are mapped to original j indices in
+// the wavefront's LDS row and each written at its rank among them, so the
+// list comes out ascending whatever the append order was; the nearest
+// neighbour (smallest FP64 r2 > 0, ties to the smaller j) is found on the
+// way.
 // ---------------------------------------------------------------------------
 
 constexpr int kP2I = 32;
 constexpr int kP2U = 4;   // 64-slice chunks per pass-2 load batch
 
 __global__ __launch_bounds__(256) void k_regf_pass2(
-    const JRec *__restrict__ jp_all, const ClusterDesc *__restrict__ desc,
-    int ncl, const double *__restrict__ xi_all,
-    const double *__restrict__ part_all, const int *__restrict__ cnt_all,
-    const uint16_t *__restrict__ seg_all, double *__restrict__ acc_all,
-    double *__restrict__ jrk_all, double *__restrict__ pot_all,
-    int *__restrict__ nnb_all, int *__restrict__ nnid_all,
-    int *__restrict__ list, int *__restrict__ off_all,
-    int *__restrict__ total, long long list_cap, int nnbmax,
+    const JRec *__restrict__ jps_all, const int *__restrict__ jidx_all,
+    const ClusterDesc *__restrict__ desc, int ncl,
+    const double *__restrict__ xi_all, const double *__restrict__ part_all,
+    int *__restrict__ nbc_all, const int *__restrict__ nbl_all,
+    double *__restrict__ acc_all, double *__restrict__ jrk_all,
+    double *__restrict__ pot_all, int *__restrict__ nnb_all,
+    int *__restrict__ nnid_all, int *__restrict__ list,
+    int *__restrict__ off_all, int *__restrict__ total, long long list_cap,
+    int nnbmax, int sortcap, const int *__restrict__ perm,
     unsigned *__restrict__ err) {
     __shared__ int s_tot[kP2I];
     __shared__ int s_off[kP2I];
+    extern __shared__ int s_rows[];   // [4][sortcap]: one list row per wave
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const ClusterDesc &D = desc[find_blk(desc, ncl, blockIdx.x)];
-    const int ni = D.ni, nslice = D.nslice, cap = D.cap, ntile = D.ntile;
+    const int ni = D.ni, nslice = D.nslice, cap = D.cap;
     const int ipb = D.ipb;
     const int i0 = (int)(blockIdx.x - D.blk0) * ipb;
-    const JRec *__restrict__ jp = jp_all + D.j0;
+    const JRec *__restrict__ jps = jps_all + D.j0;
+    const int *__restrict__ jidx = jidx_all + D.j0;
     const double *__restrict__ xi = xi_all + 3 * (size_t)D.i0;
     const double *__restrict__ part = part_all + 8 * D.pbase;
-    const int *__restrict__ cnt = cnt_all + D.pbase;
-    const uint16_t *__restrict__ seg = seg_all + D.sbase;
-    double *__restrict__ acc = acc_all + 3 * (size_t)D.i0;
-    double *__restrict__ jrk = jrk_all + 3 * (size_t)D.i0;
-    double *__restrict__ pot = pot_all + D.i0;
-    int *__restrict__ nnb_out = nnb_all + D.i0;
-    int *__restrict__ nnid = nnid_all ? nnid_all + D.i0 : nullptr;
-    int *__restrict__ off = off_all + D.i0;
+    int *__restrict__ nbc = nbc_all + D.ibase;
+    const int *__restrict__ nbl = nbl_all + D.sbase;
+    // Output row of local i: the caller's index (through perm when the i
+    // were sorted).
+    auto orow = [&](int i) -> size_t {
+        const int g = D.i0 + i;
+        return (size_t)(perm ? perm[g] : g);
+    };
 
     for (int r = wave; r < ipb; r += 4) {
         const int i = i0 + r;
         int tot = 0;
         if (i < ni) {
-            // kP2U chunks of 64 slices per batch: their partials and counts
-            // are loaded together, then added in slice order per lane.
+            // kP2U chunks of 64 slices per batch: their partials are loaded
+            // together, then added in slice order per lane.
             double sum[7] = {0, 0, 0, 0, 0, 0, 0};
-            const int *ci = cnt + (size_t)i * nslice;
-            int ct = 0;
             for (int b0 = 0; b0 < nslice; b0 += 64 * kP2U) {
                 double v[kP2U][7];
-                int c[kP2U];
 #pragma unroll
                 for (int u = 0; u < kP2U; ++u) {
                     const int q = b0 + 64 * u + lane;
@@ -806,11 +952,9 @@ __global__ __launch_bounds__(256) void k_regf_pass2(
                         const double *p = part + ((size_t)i * nslice + q) * 8;
 #pragma unroll
                         for (int k = 0; k < 7; ++k) v[u][k] = p[k];
-                        c[u] = ci[q];
                     } else {
 #pragma unroll
                         for (int k = 0; k < 7; ++k) v[u][k] = 0.0;
-                        c[u] = 0;
                     }
                 }
 #pragma unroll
@@ -819,23 +963,24 @@ __global__ __launch_bounds__(256) void k_regf_pass2(
 #pragma unroll
                         for (int k = 0; k < 7; ++k) sum[k] += v[u][k];
                     }
-                    ct += c[u];
                 }
             }
 #pragma unroll
             for (int k = 0; k < 7; ++k)
                 for (int o = 32; o >= 1; o >>= 1)
                     sum[k] += __shfl_xor(sum[k], o);
-            tot = wave_sum(ct);
+            tot = nbc[i];
             if (lane == 0) {
-                acc[3 * (size_t)i + 0] = sum[0];
-                acc[3 * (size_t)i + 1] = sum[1];
-                acc[3 * (size_t)i + 2] = sum[2];
-                jrk[3 * (size_t)i + 0] = sum[3];
-                jrk[3 * (size_t)i + 1] = sum[4];
-                jrk[3 * (size_t)i + 2] = sum[5];
-                pot[i] = sum[6];
-                nnb_out[i] = (tot <= nnbmax) ? tot : -tot;
+                const size_t q = orow(i);
+                acc_all[3 * q + 0] = sum[0];
+                acc_all[3 * q + 1] = sum[1];
+                acc_all[3 * q + 2] = sum[2];
+                jrk_all[3 * q + 0] = sum[3];
+                jrk_all[3 * q + 1] = sum[4];
+                jrk_all[3 * q + 2] = sum[5];
+                pot_all[q] = sum[6];
+                nnb_all[q] = (tot <= nnbmax) ? tot : -tot;
+                nbc[i] = 0;
             }
         }
         if (lane == 0) s_tot[r] = tot;
@@ -858,69 +1003,41 @@ __global__ __launch_bounds__(256) void k_regf_pass2(
     }
     __syncthreads();
 
+    int *__restrict__ row = s_rows + wave * sortcap;
     for (int r = wave; r < ipb; r += 4) {
         const int i = i0 + r;
         if (i >= ni) break;
         const int o = s_off[r];
         int bestj = -1;
         if (o >= 0) {
-            const int *ci = cnt + (size_t)i * nslice;
+            const int n = s_tot[r];
+            const int *__restrict__ ent = nbl + (size_t)i * cap;
             const double xa = xi[3 * (size_t)i + 0];
             const double ya = xi[3 * (size_t)i + 1];
             const double za = xi[3 * (size_t)i + 2];
             double br2 = INFINITY;
             int bj = INT_MAX;
-            int base = o;
-            // kP2U chunks of 64 slices per batch: counts loaded together,
-            // list offsets by wave scans, then the k-th entry of every
-            // chunk's segment (and its j record) loaded together.  The
-            // nearest neighbour is a strict (r2, j) minimum, so the order of
-            // the updates does not matter.
-            for (int b0 = 0; b0 < nslice; b0 += 64 * kP2U) {
-                int cq[kP2U], d0[kP2U];
-#pragma unroll
-                for (int u = 0; u < kP2U; ++u) {
-                    const int q = b0 + 64 * u + lane;
-                    cq[u] = q < nslice ? ci[q] : 0;
-                }
-                int mc = 0;
-#pragma unroll
-                for (int u = 0; u < kP2U; ++u) {
-                    const int incl = wave_incl_scan(cq[u], lane);
-                    d0[u] = base + (incl - cq[u]);
-                    base += __shfl(incl, 63);
-                    mc = cq[u] > mc ? cq[u] : mc;
-                }
-                for (int o2 = 32; o2 >= 1; o2 >>= 1) {
-                    const int t = __shfl_xor(mc, o2);
-                    mc = t > mc ? t : mc;
-                }
-                for (int k = 0; k < mc; ++k) {
-                    int jv[kP2U];
-#pragma unroll
-                    for (int u = 0; u < kP2U; ++u) {
-                        jv[u] = -1;
-                        if (k < cq[u]) {
-                            const int q = b0 + 64 * u + lane;
-                            const int js =
-                                (int)(((long long)ntile * q) / nslice) * kTile;
-                            jv[u] = js + (int)seg[((size_t)i * nslice + q) * cap + k];
-                        }
-                    }
-#pragma unroll
-                    for (int u = 0; u < kP2U; ++u) {
-                        if (jv[u] < 0) continue;
-                        const int j = jv[u];
-                        list[d0[u] + k] = j;
-                        const JRec &J = jp[j];
-                        const double r2 = sq3d(J.x - xa, J.y - ya, J.z - za);
-                        if (r2 > 0.0 && (r2 < br2 || (r2 == br2 && j < bj))) {
-                            br2 = r2;
-                            bj = j;
-                        }
-                    }
+            for (int e = lane; e < n; e += 64) {
+                const int p = ent[e];
+                const int j = jidx[p];
+                row[e] = j;
+                const JRec &J = jps[p];
+                const double r2 = sq3d(J.x - xa, J.y - ya, J.z - za);
+                if (r2 > 0.0 && (r2 < br2 || (r2 == br2 && j < bj))) {
+                    br2 = r2;
+                    bj = j;
                 }
             }
+            // Every lane of this wave reads the whole row below: the row's
+            // LDS writes complete first (the lanes run in step).
+            __threadfence_block();
+            for (int e = lane; e < n; e += 64) {
+                const int v = row[e];
+                int rank = 0;
+                for (int k = 0; k < n; ++k) rank += row[k] < v ? 1 : 0;
+                list[o + rank] = v;
+            }
+            __threadfence_block();
             // Lexicographic (r2, j) minimum over the wave.
             for (int w = 32; w >= 1; w >>= 1) {
                 const double r2o = __shfl_xor(br2, w);
@@ -933,8 +1050,9 @@ __global__ __launch_bounds__(256) void k_regf_pass2(
             bestj = (bj == INT_MAX) ? -1 : bj;
         }
         if (lane == 0) {
-            off[i] = o;
-            if (nnid) nnid[i] = bestj;
+            const size_t q = orow(i);
+            off_all[q] = o;
+            if (nnid_all) nnid_all[q] = bestj;
         }
     }
 }
@@ -1099,16 +1217,17 @@ void launch_zero_int(hipStream_t st, int *p, int n) {
 template <bool MFLAG, int G>
 static void launch_pass1(hipStream_t st, const RegfLaunch &L, long long nblk) {
     hipLaunchKernelGGL((k_regf_pass1<MFLAG, G>), dim3((unsigned)nblk), dim3(64),
-                       0, st, L.jp, L.desc, L.ncl, L.ntask, L.jst, L.xi, L.vi,
-                       L.h2, L.dtr, L.part, L.cnt, L.seg, L.err);
+                       0, st, L.j32, L.tiles, L.jps, L.jcl, L.desc, L.ncl,
+                       L.ntask, L.jst, L.xi, L.vi, L.h2, L.dtr, L.part, L.nbc,
+                       L.nbl, L.err);
 }
 
 template <bool MFLAG, int G, int NR>
 static void launch_pass1_f64(hipStream_t st, const RegfLaunch &L,
                              long long nblk) {
     hipLaunchKernelGGL((k_regf_pass1_f64<MFLAG, G, NR>), dim3((unsigned)nblk),
-                       dim3(64), 0, st, L.jp, L.desc, L.ncl, L.ntask, L.jst,
-                       L.xi, L.vi, L.h2, L.dtr, L.part, L.cnt, L.seg, L.err);
+                       dim3(64), 0, st, L.jps, L.desc, L.ncl, L.ntask, L.jst,
+                       L.xi, L.vi, L.h2, L.dtr, L.part, L.nbc, L.nbl, L.err);
 }
 
 void launch_regf_pass1(hipStream_t st, const RegfLaunch &L) {
@@ -1140,10 +1259,16 @@ void launch_regf_pass1(hipStream_t st, const RegfLaunch &L) {
 
 void launch_regf_pass2(hipStream_t st, const RegfLaunch &L) {
     if (L.nblk2 <= 0) return;
-    hipLaunchKernelGGL(k_regf_pass2, dim3((unsigned)L.nblk2), dim3(256), 0, st,
-                       L.jp, L.desc, L.ncl, L.xi, L.part, L.cnt, L.seg, L.acc,
-                       L.jrk, L.pot, L.nnb, L.nnid, L.list, L.off, L.total,
-                       L.list_cap, L.nnbmax, L.err);
+    const int sortcap = pass2_sortcap(L.nnbmax);
+    hipLaunchKernelGGL(k_regf_pass2, dim3((unsigned)L.nblk2), dim3(256),
+                       4 * sortcap * sizeof(int), st, L.jps, L.jidx, L.desc,
+                       L.ncl, L.xi, L.part, L.nbc, L.nbl, L.acc, L.jrk, L.pot,
+                       L.nnb, L.nnid, L.list, L.off, L.total, L.list_cap,
+                       L.nnbmax, sortcap, L.perm, L.err);
+}
+
+int pass2_sortcap(int nnbmax) {
+    return std::max(64, (nnbmax + 64) / 64 * 64);
 }
 
 int pass2_ipb(int ni) {

@@ -3,15 +3,17 @@
 TEST INFRASTRUCTURE ONLY (used by tests/ and __graft_entry__.smoke()).
 
 Bar: neighbour lists and nearest-neighbour indices identical to the oracle's
-(bit-exact, including order and overflow codes); acceleration and potential
-within FORCE_TOL = 1e-6 relative error per particle (|delta| / |oracle|).
-Jerk: within FORCE_TOL relative to max(|jerk|, J/KAPPA), J = sum over the
-far-field pairs of |jerk term|.  A jerk sum can cancel (J/|jerk| reaches
-~150 in these clusters), and the FP32 pair arithmetic the engine uses by
-design bounds the error by the per-term error times J: about 8 * 2^-24 * J
-in the worst case (1 ulp v_rsq_f32 cubed, FP32 products).  KAPPA = 2 is the
-cancellation ratio at which that bound equals FORCE_TOL; below it the error
-is measured against |jerk| itself.
+(bit-exact, including order and overflow codes); acceleration, jerk and
+potential within FORCE_TOL = 1e-6 relative error per particle, normwise
+(|delta_i| / |oracle_i|).
+
+The condition-scaled jerk error |delta_i| / max(|jerk_i|, J_i / KAPPA), with
+J_i the sum over the far-field pairs of |jerk term|, is reported next to it:
+a jerk sum can cancel (J / |jerk| reaches ~160 in these clusters), which is
+why the engine evaluates the pairs of the near band in FP64.  Only a call
+whose i have no neighbour sphere to anchor that band (h2 = 0, where the far
+field includes the closest pairs) is held to the condition-scaled bar
+instead, with jerk_plain=False.
 """
 import numpy as np
 
@@ -47,8 +49,10 @@ def jerk_scale(m, xj, vj, xi, vi, lst):
     return out
 
 
-def assert_parity(g, o, tol=FORCE_TOL, label="", inputs=None):
-    """inputs = (m, xj, vj, xi, vi) of the call, for the jerk scale."""
+def assert_parity(g, o, tol=FORCE_TOL, label="", inputs=None,
+                  jerk_plain=True):
+    """inputs = (m, xj, vj, xi, vi) of the call, for the condition-scaled
+    jerk report (and the bar when jerk_plain is False)."""
     acc, jrk, pot, lst, nnid = g
     oacc, ojrk, opot, olst, onnid = o
     assert np.array_equal(lst[:, 0], olst[:, 0]), f"{label}: counts differ"
@@ -67,6 +71,7 @@ def assert_parity(g, o, tol=FORCE_TOL, label="", inputs=None):
     msg = (f"{label}: rel err acc max {ea.max():.2e} med {np.median(ea):.2e}; "
            f"jrk max {ej.max():.2e} med {np.median(ej):.2e}; "
            f"pot max {ep.max():.2e}")
+    ejs = None
     if inputs is not None:
         m, xj, vj, xi, vi = inputs
         J = jerk_scale(m, xj, vj, xi[ok], vi[ok], olst[ok])
@@ -76,11 +81,11 @@ def assert_parity(g, o, tol=FORCE_TOL, label="", inputs=None):
         ejs = np.where(scale > 0, dj / np.where(scale > 0, scale, 1.0), dj)
         msg += (f"; jrk vs max(|j|,J/{KAPPA:g}) max {ejs.max():.2e}; "
                 f"cancellation J/|j| max {np.max(J / np.maximum(nj, 1e-300)):.1f}")
-    else:
-        ejs = ej
     print(msg)
     assert ea.max() < tol, f"{label}: acc {ea.max()}"
     assert ep.max() < tol, f"{label}: pot {ep.max()}"
-    assert ejs.max() < tol, f"{label}: jrk {ejs.max()}"
-
-
+    if jerk_plain:
+        assert ej.max() < tol, f"{label}: jrk {ej.max()}"
+    else:
+        assert ejs is not None, "the condition-scaled bar needs the inputs"
+        assert ejs.max() < tol, f"{label}: jrk (condition-scaled) {ejs.max()}"

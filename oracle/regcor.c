@@ -41,6 +41,7 @@ typedef struct {
     double time, rscale, rdens[3], rc2, rsfac, znbmin, smax, dtmin, etar,
         smin, bodym, tidal4;
     int nnbopt, nnbmax, n, ntot, ifirst, lmax, nb_flag, m_flag;
+    int kz13, kz14, kz37, kz38;   /* must be 0 (not restated) */
 } orc_regcor_par;
 
 /* Per-body arrays (0-based body p = I - 1; vectors as 3 consecutive

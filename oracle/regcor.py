@@ -31,7 +31,9 @@ class Par(ctypes.Structure):
                 ("nnbopt", ctypes.c_int), ("nnbmax", ctypes.c_int),
                 ("n", ctypes.c_int), ("ntot", ctypes.c_int),
                 ("ifirst", ctypes.c_int), ("lmax", ctypes.c_int),
-                ("nb_flag", ctypes.c_int), ("m_flag", ctypes.c_int)]
+                ("nb_flag", ctypes.c_int), ("m_flag", ctypes.c_int),
+                ("kz13", ctypes.c_int), ("kz14", ctypes.c_int),
+                ("kz37", ctypes.c_int), ("kz38", ctypes.c_int)]
 
 
 _dp = ctypes.POINTER(ctypes.c_double)

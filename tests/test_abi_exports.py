@@ -38,12 +38,12 @@ def test_python_front_end_binds_every_symbol():
 
 
 def test_regcor_structures_match_the_header_layout():
-    """gpunb_regcor_par: 14 doubles then 8 ints; gpunb_regcor_state: 22
+    """gpunb_regcor_par: 14 doubles then 12 ints; gpunb_regcor_state: 22
     pointers (the C side asserts the same sizes, gpunb_iface.f90 declares
     the same members in the same order)."""
     import ctypes
     from nbody6ppgpu_amd.gpunb import RegcorPar, RegcorState
-    assert ctypes.sizeof(RegcorPar) == 14 * 8 + 8 * 4
+    assert ctypes.sizeof(RegcorPar) == 14 * 8 + 12 * 4
     assert RegcorPar.nnbopt.offset == 14 * 8
     assert ctypes.sizeof(RegcorState) == 22 * ctypes.sizeof(ctypes.c_void_p)
     assert [f for f, _ in RegcorState._fields_][-2:] == ["h", "list"]

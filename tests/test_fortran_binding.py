@@ -140,3 +140,85 @@ def test_fortran_driver_matches_c_abi(tmp_path):
     scale = np.linalg.norm(oa, axis=1) + 1e-300
     assert np.all(np.linalg.norm(firr - oa, axis=1) <= 1e-10 * scale + 1e-12 * np.max(scale))
     del nn
+
+
+HOST_LOOP = os.path.join(HERE, "fortran", "_build", "host_loop")
+
+
+def run_host_loop(tmp_path, c, ifirst, rs, stepr, ireg, lmax, nnbmax, nnbopt,
+                  m_flag, nb_flag):
+    ntot = c.n
+    bodym = float(np.mean(c.m))
+    fin, fout = tmp_path / "hl_in.bin", tmp_path / "hl_out.bin"
+    with open(fin, "wb") as f:
+        np.array([ntot, ifirst, lmax, nnbmax, nnbopt, m_flag, nb_flag,
+                  len(ireg)], dtype=np.int32).tofile(f)
+        np.array([bodym], dtype=np.float64).tofile(f)
+        for a in (c.m, c.x, c.v, rs, stepr):
+            np.ascontiguousarray(a, dtype=np.float64).tofile(f)
+        np.ascontiguousarray(ireg, dtype=np.int32).tofile(f)
+    env = dict(os.environ, GPUNB_QUIET="1")
+    subprocess.run([HOST_LOOP, str(fin), str(fout)], check=True, env=env,
+                   timeout=120)
+    raw = open(fout, "rb").read()
+    off = 0
+
+    def take(dtype, count):
+        nonlocal off
+        a = np.frombuffer(raw, dtype=dtype, count=count, offset=off)
+        off += a.nbytes
+        return a
+    rs_out = take(np.float64, ntot)
+    phi = take(np.float64, ntot)
+    freg = take(np.float64, 3 * ntot).reshape(ntot, 3)
+    fdr = take(np.float64, 3 * ntot).reshape(ntot, 3)
+    lst = take(np.int32, lmax * ntot).reshape(ntot, lmax)
+    nretry = int(take(np.int32, 1)[0])
+    assert off == len(raw)
+    return rs_out, phi, freg, fdr, lst, nretry, bodym
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("m_flag,nb_flag", [(0, 1), (1, 0)])
+def test_fortran_host_loop_overflow_retry(tmp_path, m_flag, nb_flag):
+    """The util_gpu.F:25-112 call sequence from Fortran: 1024-i chunks of
+    an active list, overflow -> shrink RS and redo the chunk, IFIRST offset
+    and self removal, PHIDBL.  Every accepted row equals the oracle's list
+    for the body's final RS (plus IFIRST, without the body), and PHIDBL,
+    FREG, FDR meet the plain bar against the oracle at the same radii."""
+    from oracle import oracle
+    from oracle.parity import FORCE_TOL, rel_err
+    assert os.path.exists(HOST_LOOP), "build the Fortran drivers (build())"
+    m = ics.kroupa(4000, seed=312) if m_flag else None
+    c = ics.plummer(4000, seed=311, m=m)
+    ifirst = 21                  # bodies 1..20 stand for KS components
+    rs, stepr = ics.neighbour_radii(c, k=48)
+    rng = np.random.default_rng(313)
+    ireg = np.sort(rng.choice(np.arange(ifirst, c.n + 1), 2500,
+                              replace=False)).astype(np.int32)
+    lmax, nnbmax, nnbopt = 64, 60, 40
+    rs_out, phi, freg, fdr, lst, nretry, bodym = run_host_loop(
+        tmp_path, c, ifirst, rs, stepr, ireg, lmax, nnbmax, nnbopt, m_flag,
+        nb_flag)
+    assert nretry > 0, "the radii were chosen so that some lists overflow"
+    shrunk = rs_out < rs
+    assert np.all(rs_out <= rs) and shrunk.sum() > 0
+    assert np.array_equal(rs_out[~shrunk], rs[~shrunk])
+    js = slice(ifirst - 1, None)
+    idx = ireg - 1
+    h2 = rs_out[idx] ** 2
+    if m_flag:
+        h2 = h2 / bodym
+    o = oracle.regf(c.m[js], c.x[js], c.v[js], h2, stepr[idx], c.x[idx],
+                    c.v[idx], lmax, nnbmax, m_flag)
+    assert np.all(o[3][:, 0] >= 0) and np.all(o[3][:, 0] <= nnbmax)
+    for r, i in enumerate(idx):
+        k = o[3][r, 0]
+        want = o[3][r, 1:1 + k] + ifirst
+        want = want[want != i + 1]
+        assert lst[i, 0] == len(want), f"row of body {i + 1}"
+        assert np.array_equal(lst[i, 1:1 + len(want)], want), f"body {i + 1}"
+    for got, ref in ((freg[idx], o[0]), (fdr[idx], o[1]), (phi[idx], o[2])):
+        assert rel_err(got, ref).max() < FORCE_TOL
+    inactive = np.setdiff1d(np.arange(c.n), idx)
+    assert np.all(lst[inactive] == 0) and np.all(phi[inactive] == 0)

@@ -186,6 +186,19 @@ def test_invalid_bodies_are_reported_and_left_unchanged():
             regcor.correct_host(bad, dev2, block["idx"], block["xi"],
                                 block["vi"], block["freg"], block["fdr"],
                                 block["nnb"], block["off"], block["lst"])
+        # Options whose REGCOR_GPU branches are not on the device are
+        # refused, nothing changed.
+        before = dev2.numpy()
+        for kz in ("kz13", "kz14", "kz37", "kz38"):
+            bad = regcor.make_par(**dict(par_kw, **{kz: 1}))
+            with pytest.raises(gpunb.GpunbError, match="KZ"):
+                regcor.correct_host(bad, dev2, block["idx"], block["xi"],
+                                    block["vi"], block["freg"],
+                                    block["fdr"], block["nnb"], block["off"],
+                                    block["lst"])
+        after = dev2.numpy()
+        for k in before:
+            assert np.array_equal(after[k], before[k]), k
     finally:
         gpunb.gpunb_close()
 

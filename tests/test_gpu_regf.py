@@ -85,7 +85,10 @@ def test_zero_radius_and_lmax_stride():
     args = (c.m, c.x, c.v, z, z, c.x, c.v, 7, 3, 0)
     g = gpu_regf(*args)
     assert np.all(g[3][:, 0] == 0) and np.all(np.isfinite(g[0]))
-    assert_parity(g, oracle.regf(*args), label="h2=0", inputs=args[:3] + args[5:7])
+    # h2 = 0: no neighbour sphere anchors the FP64 near band, the closest
+    # pairs are in the far field: jerk at the condition-scaled bar.
+    assert_parity(g, oracle.regf(*args), label="h2=0", inputs=args[:3] + args[5:7],
+                  jerk_plain=False)
     h2, dtr = setup(c, 20)
     args = (c.m, c.x, c.v, h2, dtr, c.x, c.v, 200, 60, 0)
     assert_parity(gpu_regf(*args), oracle.regf(*args), label="stride",

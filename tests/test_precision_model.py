@@ -8,8 +8,8 @@ FP64 every 64 j).  Compared with the FP64 oracle it shows:
   jerk sums cancel (sum |terms| / |jerk| reaches ~160), so even rounding only
   1/r to FP32 with everything else in FP64 misses the bar, and the full FP32
   pair path misses it several times over;
-* evaluating the near band -- min(r2, q) < 9 h2 w_j, w_j = max(1, x^(2/3),
-  1.4 x / 9), x = m_j / <m> -- in FP64, as the engine does, brings acc, jerk
+* evaluating the near band -- min(r2, q) < 12 h2 w_j, w_j = max(1, x^(2/3),
+  1.4 x / 12), x = m_j / <m> -- in FP64, as the engine does, brings acc, jerk
   and pot below 1e-6 with a margin of two, also with the tile-relative
   differences the engine forms;
 * the band is a small fraction of the pairs at the benchmark size.
@@ -30,7 +30,7 @@ from oracle import oracle
 from oracle.parity import FORCE_TOL
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-BAND = 9.0
+BAND = 12.0
 _dp = ctypes.POINTER(ctypes.c_double)
 
 
@@ -142,7 +142,7 @@ def test_band_meets_bar_config1(model, m_flag):
     for mode in (0, 1):
         (ea, ej, ep), _ = errors(model, c, h2, dtr, np.arange(c.n), m_flag,
                                  mode, BAND)
-        assert max(ea, ej, ep) < FORCE_TOL
+        assert max(ea, ej, ep) < FORCE_TOL / 2
 
 
 @pytest.mark.parametrize("m_flag", [0, 1])
@@ -151,12 +151,12 @@ def test_band_meets_bar_config2(model, m_flag):
     idx = np.random.default_rng(11 + m_flag).choice(c.n, 6144, replace=False)
     (ea, ej, ep), frac = errors(model, c, h2, dtr, idx, m_flag, 1, BAND)
     assert max(ea, ej, ep) < FORCE_TOL / 2
-    assert frac < 0.25
+    assert frac < 0.35
 
 
 def test_band_meets_bar_king_sample(model):
     """King W0=6 N=131072 (the benchmark workload): 24 random tiles' worth
-    of i; the FP64 band is about 2% of the pairs."""
+    of i; the FP64 band is about 3% of the pairs."""
     n = 131072
     c = ics.king(n, 6.0, seed=203, m=ics.kroupa(n, seed=204))
     rs, dtr = ics.neighbour_radii(c, k=48)
@@ -164,4 +164,4 @@ def test_band_meets_bar_king_sample(model):
     idx = np.random.default_rng(13).choice(n, 1536, replace=False)
     (ea, ej, ep), frac = errors(model, c, h2, dtr, idx, 0, 1, BAND)
     assert max(ea, ej, ep) < FORCE_TOL / 2
-    assert frac < 0.03
+    assert frac < 0.04
