@@ -69,9 +69,13 @@ struct JClus {
     double pad;
 };
 
-// Near band of the FP32 pair loop: pairs with min(r2, q) < K_BAND * h2 * w_j
-// (times <m> with mass-weighted lists) are evaluated in FP64.
+// Near band of the FP32 pair loop: a tile holding a pair with min(r2, q) <
+// K_BAND * h2 * w_j (times <m> with mass-weighted lists) for some lane of a
+// wave is evaluated in FP64 by that wave.
 constexpr float kBand = 12.0f;
+// j heavier than kHeavy <m> go to tiles of their own (after the light j of
+// their cluster): light tiles then carry w_j <= max(1, kHeavy^(2/3)) ~ 2.
+constexpr float kHeavy = 2.8f;
 // Tile floor of the near band: the tile-relative FP32 difference of a pair
 // carries up to 2u (|dx| + e_t) per component, so pairs with |dx| below
 // kTileBand e_t are also evaluated in FP64 (this matters only where the

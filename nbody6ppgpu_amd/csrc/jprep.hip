@@ -12,8 +12,9 @@
 //              in a fixed order
 //   k_jstat2   per cluster: the blocks' partials in block order -> JClus
 //              (bounding box, mean mass)
-//   k_jkey     64-bit keys (cluster << 32 | 30-bit Morton code of the
-//              position in the cluster's box; padding records sort last)
+//   k_jkey     64-bit keys (cluster << 32 | heavy << 30 | 30-bit Morton code
+//              of the position in the cluster's box; padding records sort
+//              last)
 //   rocprim radix sort of (key, original index): stable, so equal codes keep
 //              their index order
 //   k_jtile    one wavefront per tile: sorted FP64 records, sorted -> original
@@ -182,6 +183,10 @@ __global__ __launch_bounds__(256) void k_jkey(
         code = spread3(cell10(r.x, J.lo[0], scale)) |
                (spread3(cell10(r.y, J.lo[1], scale)) << 1) |
                (spread3(cell10(r.z, J.lo[2], scale)) << 2);
+        // Heavy bodies (m > kHeavy <m>) sort after the light ones, in tiles
+        // of their own: their band weight w_j would otherwise widen the
+        // near region of every tile that holds one.
+        if (r.m > (double)kHeavy * J.mbar) code |= 1u << 30;
     }
     key[g] = ((unsigned long long)c << 32) | code;
     val[g] = g;

@@ -56,13 +56,6 @@ __device__ inline int wave_sum(int v) {
     for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
     return v;
 }
-__device__ inline int wave_incl_scan(int v, int lane) {
-    for (int o = 1; o < 64; o <<= 1) {
-        int u = __shfl_up(v, o);
-        if (lane >= o) v += u;
-    }
-    return v;
-}
 
 __device__ inline void set_pad(JRec &r) {
     r.x = kPadPos;
@@ -310,65 +303,87 @@ __global__ void k_zero_int(int *p, int n) {
     if (k < n) p[k] = 0;
 }
 
+// Threshold of the FP64 screen min(r2, r2 + 2 dtr dx.dv) < thr (times m_j
+// with mass-weighted lists) for a lane with h2 > 0: pairs below it are
+// decided by the oracle's exact predicate.  The FP32-form screen's value in
+// FP64 is within 4.1u S <= 4.1u H (1 + 2 rho)^2 of its exact value (u =
+// 2^-53, rho = |dtr| (|v_i| + max|v_j|) / sqrt(H_min)), and the oracle's r2p
+// (separate multiply and add in the prediction) within 3.1u H + 4.1u H (1 +
+// 2 rho) of the exact one near the threshold; the threshold allows twice
+// their sum.
+template <bool MFLAG>
+__device__ inline double f64_screen_thr(double h2d, double dtrd, double u0,
+                                        double w0, double q0,
+                                        const JStats *__restrict__ jst) {
+    const double vmax = bitsd(jst->vmax_bits);
+    double hmin = h2d;
+    if (MFLAG) {
+        const double mmin = bitsd(jst->mmin_bits);
+        hmin = (mmin < INFINITY) ? h2d * mmin : h2d;
+    }
+    const double u = 1.1102230246251565e-16;  // 2^-53
+    const double rho = fabs(dtrd) * (speed_up(u0, w0, q0) + vmax) / sqrt(hmin);
+    const double g = 1.0 + 2.0 * rho;
+    const double eps =
+        2.0 * (8.0 * u * g * g + 4.0 * u) + (MFLAG ? 24.0 : 16.0) * u;
+    const double tt = h2d + h2d * eps;
+    return fmax(nextafter(nextafter(tt, INFINITY), INFINITY), 1e-30);
+}
+
 // ---------------------------------------------------------------------------
 // Regular force, pass 1: one wavefront = 64 i-particles (one per lane) x one
-// contiguous slice of the Morton-sorted j set.  j records are wave-uniform,
-// so they arrive through the scalar cache as SGPR operands.
+// contiguous slice of the Morton-sorted j set, tile by tile (64 j).  j
+// records are wave-uniform, so they arrive through the scalar cache as SGPR
+// operands.  Each tile is run in one of two ways, chosen per wave:
 //
-// Pair differences.  Every j is stored as an FP32 offset o_j from the FP64
-// centre C_t of its tile (64 spatially close j), so per (lane, tile) the
-// FP64 difference C_t - x_i is formed once and rounded to FP32 (D), and per
-// pair dx = D + o_j is one FP32 add per component (velocities alike).  With
-// |o_j| <= e_t (the tile's offset bound) the rounding error is, per
-// component, at most 2u (|dx_c| + |o_c|) (u = 2^-24): D carries u |C - x_i|
-// <= u (|dx_c| + |o_c|), o_j u |o_c|, the add u |dx_c|.
+// Far tiles: FP32 pair terms on tile-relative differences.  Every j is
+// stored as an FP32 offset o_j from the FP64 centre C_t of its tile, so per
+// (lane, tile) the FP64 difference C_t - x_i is formed once and rounded to
+// FP32 (D), and per pair dx = D + o_j is one FP32 add per component
+// (velocities alike).  With |o_j| <= e_t (the tile's offset bound) the
+// rounding error is, per component, at most 2u (|dx_c| + |o_c|) (u = 2^-24).
+// The FP32 sums of a tile are folded into FP64 accumulators at its end.
 //
-// Screen.  A pair is flagged when the FP32 screen value s = min(r2, q),
-// q = r2 + 2 dtr dx.dv (a lower bound of the predicted distance r2p = q +
-// dtr^2 |dv|^2, so min(r2, q) <= min(r2, r2p)), lies below T_j = B w_j:
-//   B   = K h2 (K h2 <m> with mass-weighted lists), K = kBand, per lane
-//   w_j = max(1, x^(2/3), 1.4 x / K), x = m_j / <m>, per j,
-// and B is raised per tile to (e_t / 2)^2 (e_t: the tile's offset bound), so
-// pairs whose tile-relative difference would carry a large relative error
-// (|dx| not well above e_t) are among them.
-// Flagged pairs are the near band: the list candidates and the pairs close
-// enough (relative to the neighbour sphere, and more so for heavy j whose
-// terms are large) that their FP32 rounding would dominate the jerk error.
-// They leave the FP32 sums and go to the lane's queue in LDS; at the end of
-// a tile after which a lane's queue might overflow, and after the last
-// tile, the wave evaluates every queued pair in FP64 (lanes in step, each
-// on its own queue): the oracle's neighbour predicate bit for bit (a
-// neighbour is appended to the lane's segment in j order), otherwise the
-// FP64 force terms (1/r by v_rsq_f64 and one Newton step) into the FP64 sums.
+// Near tiles: the FP64 pair loop of k_regf_pass1_f64 (FP64 differences of
+// the sorted FP64 records, 1/r by v_rsq_f64 and one Newton step, the
+// oracle's neighbour predicate bit for bit on the pairs its screen cannot
+// decide), accumulating into the FP64 sums directly.
 //
-// Why no neighbour is lost.  T_j >= 1.4 H_max with H_max = h2 (m_j h2 with
-// mass-weighted lists), so the screen misses a neighbour only if s exceeds
-// the FP64 predicate's value by 0.4 H.  Bounding the errors above for a pair
-// with min(r2, r2p) < H, with rho = |dtr| (|v_i| + max|v_j|) / sqrt(H_min):
-//   |s_32 - s| <= u H [23 (1 + rho)^2 + 1]
-//                 + 8u sqrt(H) (1 + 2 rho) (e_t + |dtr| ev_t)
-// (ev_t: the tile's velocity-offset bound), plus the FP64 predicate's own
-// rounding, below 1e-15 (1 + rho)^2 H.  A lane treats a tile as unsafe when
-// this could exceed 0.35 H; then every pair of that tile is flagged (only
-// tiles spanning many neighbour radii are affected, e.g. tiles that straddle
-// two Morton cells).
-//
-// Far tiles.  Per (lane, tile), with d = |D|, R = d - e_t and A = |dtr|
-// (|DV| + ev_t) (|DV| as d), every pair of the tile has |dx_32| >= R and
-// |dtr| |dv_32| <= A up to a few u, so s_32 >= R^2 - 2 A R; when R > A +
-// sqrt(A^2 + B wmax_t) (with a 1e-5 relative margin for the roundings) no
-// pair of the tile can be flagged, and when no lane of the wave is near the
-// tile the wave runs it without the screen.  Both loops evaluate unflagged
-// pairs with the same instructions, so the split does not change results.
-//
-// The FP32 partial sums are folded into FP64 accumulators every tile.
+// The split.  FP32 rounding of a pair term is harmless unless the pair is
+// close: relative to the neighbour sphere, and more so for heavy j whose
+// terms are large.  A pair is in the near band when min(r2, q) < B w_j,
+//   q   = r2 + 2 dtr dx.dv (a lower bound of the predicted distance
+//         r2p = q + dtr^2 |dv|^2, so min(r2, q) <= min(r2, r2p)),
+//   B   = K h2 (K h2 <m> with mass-weighted lists), K = kBand, per lane,
+//         raised per tile to (e_t / 2)^2, so that no FP32 pair has |dx|
+//         below e_t / 2 (where the tile-relative difference would carry a
+//         large relative error),
+//   w_j = max(1, x^(2/3), 1.4 x / K), x = m_j / <m>, per j.
+// Per (lane, tile), with d = |D|, R = d - e_t and A = |dtr| (|DV| + ev_t)
+// (|DV| as d, ev_t the tile's velocity-offset bound), every pair of the
+// tile has r >= R and |dtr| |dv| <= A, so min(r2, r2p) >= (R - A)^2 when
+// R > A; a lane finds the tile far when R > A + sqrt(A^2 + B_t wmax_t)
+// (wmax_t the tile's largest w, 1e-5 relative margins for the FP32
+// evaluation of the test), i.e. (R - A)^2 > B_t wmax_t: no pair of the tile
+// is in the band.  A wave runs a tile in FP32 only when the tile is far for
+// every lane, else in FP64.  Since B_t w_j >= 1.4 H (H = h2, or m_j h2 with
+// mass-weighted lists: K w_j >= 1.4 x), a far tile holds no neighbour, so
+// every list entry comes from the FP64 predicate.  Heavy j (m_j > kHeavy
+// <m>) sit in tiles of their own, so light tiles keep wmax_t <= ~2.
 // ---------------------------------------------------------------------------
 
-// Queued FP64 pairs per lane.  The queue is flushed at tile ends only (a
-// flush inside the pair loop would keep its FP64 temporaries live across
-// the loop and cost occupancy), so it holds a whole tile beyond the flush
-// threshold.
-constexpr int kQ = kTile + 16;
+// Opaque copy of a record held in SGPRs: the look-ahead loads of the pair
+// loops are loop-carried values, and without this LLVM folds the phi of the
+// pre-loop load and the in-loop load into one load at the loop top (waited
+// on at once).  Emits no instruction.
+__device__ inline void opaque_sgpr(JRec32 &r) {
+    asm volatile("" : "+s"(r.ox), "+s"(r.oy), "+s"(r.oz), "+s"(r.m),
+                      "+s"(r.ovx), "+s"(r.ovy), "+s"(r.ovz), "+s"(r.w));
+}
+__device__ inline void opaque_sgpr(JRec &r) {
+    asm volatile("" : "+s"(r.x), "+s"(r.y), "+s"(r.z), "+s"(r.vx),
+                      "+s"(r.vy), "+s"(r.vz), "+s"(r.m));
+}
 
 template <bool MFLAG, int G>
 __global__ __launch_bounds__(64) void k_regf_pass1(
@@ -380,7 +395,6 @@ __global__ __launch_bounds__(64) void k_regf_pass1(
     const double *__restrict__ dtr_all, double *__restrict__ part_all,
     int *__restrict__ nbc_all, int *__restrict__ nbl_all,
     unsigned *__restrict__ err) {
-    __shared__ uint16_t qbuf[kQ][64];
     // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
     // contiguous range of tasks so that it streams only its share of the j
     // array through its own L2.  Tasks are slice-major within a cluster.
@@ -410,14 +424,12 @@ __global__ __launch_bounds__(64) void k_regf_pass1(
 
     const int tb = (int)(((long long)ntile * s) / nslice);
     const int te = (int)(((long long)ntile * (s + 1)) / nslice);
-    const int jbeg = tb * kTile;
 
     double x0 = 0.0, y0 = 0.0, z0 = 0.0, u0 = 0.0, w0 = 0.0, q0 = 0.0;
-    double h2d = 0.0, dtrd = 0.0;
-    float dtr2x = 0.0f;         // 2 dtr
+    double h2d = 0.0, dtrd = 0.0, dtr2x = 0.0;   // dtr2x = 2 dtr
+    double thr = -1.0;          // FP64 screen threshold (k_regf_pass1_f64)
     float adtr = 0.0f;          // >= |dtr|
-    float B = -INFINITY;        // band base; inactive lanes never flag
-    float lim = -1.0f;          // e_t + |dtr| ev_t above lim: unsafe tile
+    float B = 0.0f;             // band base
     if (act) {
         x0 = xi[3 * (size_t)i + 0];
         y0 = xi[3 * (size_t)i + 1];
@@ -427,151 +439,55 @@ __global__ __launch_bounds__(64) void k_regf_pass1(
         q0 = vi[3 * (size_t)i + 2];
         h2d = h2[i];
         dtrd = dtr[i];
+        dtr2x = 2.0 * dtrd;
         const double chk = x0 + y0 + z0 + u0 + w0 + q0 + h2d + dtrd;
         if (chk != chk) atomicOr(err, ERR_NAN_I);
-        const float dtr32 = (float)dtrd;
-        dtr2x = 2.0f * dtr32;
-        adtr = fabsf(dtr32) * (1.0f + 2.0f * kU32);
-        const double u = (double)kU32;
+        adtr = fabsf((float)dtrd) * (1.0f + 2.0f * kU32);
         if (h2d > 0.0) {
             const double bb = (double)kBand * h2d * (MFLAG ? mbar : 1.0) *
-                              (1.0 + 8.0 * u);
-            B = fmaxf((float)bb * (1.0f + 2.0f * kU32), FLT_MIN);
-            // Unsafe-tile limit from the error bound above.
-            const double vmax = bitsd(jst->vmax_bits);
-            double hmin = h2d;
-            if (MFLAG) {
-                const double mmin = bitsd(jst->mmin_bits);
-                hmin = (mmin < INFINITY) ? h2d * mmin : h2d;
-            }
-            const double sh = sqrt(hmin);
-            const double rho = fabs(dtrd) * (speed_up(u0, w0, q0) + vmax) / sh;
-            const double g = 1.0 + rho;
-            const double a0 = u * (23.0 * g * g + 1.0) + 1e-15 * g * g;
-            const double a1 = 8.0 * u * (1.0 + 2.0 * rho) / sh;
-            const double slack = 0.35 - a0;
-            lim = slack > 0.0 ? (float)(slack / a1 * (1.0 - 1e-6)) : -1.0f;
+                              (1.0 + 8.0 * (double)kU32);
+            B = (float)bb * (1.0f + 2.0f * kU32);
+            thr = f64_screen_thr<MFLAG>(h2d, dtrd, u0, w0, q0, jst);
         } else {
-            // No neighbours (H = 0): only r2 == 0 pairs are screened, so the
-            // self pair never enters the FP32 sums.
-            B = FLT_MIN;
-            lim = INFINITY;
+            thr = 1e-30;
         }
     }
 
-    float ax = 0.f, ay = 0.f, az = 0.f, jx = 0.f, jy = 0.f, jz = 0.f, ph = 0.f;
     double Ax = 0.0, Ay = 0.0, Az = 0.0, Jx = 0.0, Jy = 0.0, Jz = 0.0, Ph = 0.0;
-    int qn = 0;
 
-    auto terms32 = [&](float m, float fx, float fy, float fz, float gx,
-                       float gy, float gz, float rv, float rinv) {
-        const float rinv2 = rinv * rinv;
-        const float mr = m * rinv;
-        const float mr3 = mr * rinv2;
-        const float al = (-3.0f * rv) * rinv2;
-        ph += mr;
-        ax = fmaf(mr3, fx, ax);
-        ay = fmaf(mr3, fy, ay);
-        az = fmaf(mr3, fz, az);
-        jx = fmaf(mr3, fmaf(al, fx, gx), jx);
-        jy = fmaf(mr3, fmaf(al, fy, gy), jy);
-        jz = fmaf(mr3, fmaf(al, fz, gz), jz);
-    };
-
-    // FP64 evaluation of the queued pairs of all lanes, 64 at a time.  The
-    // lanes' queues are taken as one sequence in lane order (lane l's
-    // entries at [excl_l, excl_l + qn_l)); in each batch a lane evaluates
-    // one entry for its owner, whose i data it fetches from the owner lane.
-    // An owner's entries form one contiguous run per batch, so a segmented
-    // suffix sum leaves the run's sums in its first lane, from which the
-    // owner adds them to its FP64 sums: a fixed order, whatever the lanes'
-    // queue lengths.  Neighbours found are appended to the owner's list.
-    auto flush = [&]() {
-        const int incl = wave_incl_scan(qn, lane);
-        const int excl = incl - qn;
-        const int total = __builtin_amdgcn_readfirstlane(__shfl(incl, 63));
-        for (int b0 = 0; b0 < total; b0 += 64) {
-            const int e = b0 + lane;
-            // Owner of entry e: the last lane o with excl_o <= e.
-            int o = 0;
-#pragma unroll
-            for (int step = 32; step >= 1; step >>= 1) {
-                const int c2 = o + step;
-                if (__shfl(excl, c2) <= e) o = c2;
-            }
-            const bool valid = e < total;
-            const int k = e - __shfl(excl, o);
-            const int pl = qbuf[valid ? k : 0][valid ? o : 0];
-            const int p = jbeg + pl;
-            const double ox = __shfl(x0, o), oy = __shfl(y0, o), oz = __shfl(z0, o);
-            const double ou = __shfl(u0, o), ow = __shfl(w0, o), oq = __shfl(q0, o);
-            const double oh2 = __shfl(h2d, o), odt = __shfl(dtrd, o);
-            double r[7] = {0, 0, 0, 0, 0, 0, 0};
-            if (valid) {
-                const JRec J = jps[p];
-                const double dx = J.x - ox, dy = J.y - oy, dz = J.z - oz;
-                const double dvx = J.vx - ou, dvy = J.vy - ow, dvz = J.vz - oq;
-                const double r2d = sq3d(dx, dy, dz);
-                const double pxd = dx + odt * dvx;
-                const double pyd = dy + odt * dvy;
-                const double pzd = dz + odt * dvz;
-                const double r2pd = sq3d(pxd, pyd, pzd);
-                const double mnd = r2d < r2pd ? r2d : r2pd;
-                const double H = MFLAG ? J.m * oh2 : oh2;
-                if ((mnd < H) && (p < nj)) {
-                    // The owner's list: slices append in any order (pass 2
-                    // sorts); entries beyond cap only count (overflow).
-                    const int io = iw * 64 + o;
-                    const int k2 = atomicAdd(&nbc[io], 1);
-                    if (k2 < cap) nbl[(size_t)io * cap + k2] = p;
-                } else if (r2d != 0.0) {
-                    double y = __builtin_amdgcn_rsq(r2d);
-                    y = fma(y, 0.5 * fma(-r2d * y, y, 1.0), y);
-                    const double rinv2 = y * y;
-                    const double mr = J.m * y;
-                    const double mr3 = mr * rinv2;
-                    const double rv = fma(dz, dvz, fma(dy, dvy, dx * dvx));
-                    const double al = -3.0 * rv * rinv2;
-                    r[0] = mr3 * dx;
-                    r[1] = mr3 * dy;
-                    r[2] = mr3 * dz;
-                    r[3] = mr3 * fma(al, dx, dvx);
-                    r[4] = mr3 * fma(al, dy, dvy);
-                    r[5] = mr3 * fma(al, dz, dvz);
-                    r[6] = mr;
-                }
-            }
-            // Segmented suffix sums over runs of equal owner.  Every shuffle
-            // runs with all lanes active (a permute reads 0 from a lane
-            // outside the active mask, so it must not sit under a
-            // lane-dependent condition).
-#pragma unroll
-            for (int d = 1; d < 64; d <<= 1) {
-                const int od = __shfl_down(o, d);
-                const bool same = (lane + d < 64) & (od == o);
-#pragma unroll
-                for (int c = 0; c < 7; ++c) {
-                    const double t = __shfl_down(r[c], d);
-                    if (same) r[c] += t;
-                }
-            }
-            // The owner's run in this batch starts at lane max(excl - b0, 0).
-            const bool mine = qn > 0 && incl > b0 && excl < b0 + 64;
-            const int head = excl > b0 ? excl - b0 : 0;
-            double g[7];
-#pragma unroll
-            for (int c = 0; c < 7; ++c) g[c] = __shfl(r[c], head);
-            if (mine) {
-                Ax += g[0];
-                Ay += g[1];
-                Az += g[2];
-                Jx += g[3];
-                Jy += g[4];
-                Jz += g[5];
-                Ph += g[6];
-            }
+    // FP64 pair of a near tile: the screen of k_regf_pass1_f64, the exact
+    // predicate for the pairs it flags.
+    auto resolve = [&](const JRec &J, int j) {
+        const double dx = J.x - x0, dy = J.y - y0, dz = J.z - z0;
+        const double dvx = J.vx - u0, dvy = J.vy - w0, dvz = J.vz - q0;
+        const double r2d = sq3d(dx, dy, dz);
+        const double pxd = dx + dtrd * dvx;
+        const double pyd = dy + dtrd * dvy;
+        const double pzd = dz + dtrd * dvz;
+        const double r2pd = sq3d(pxd, pyd, pzd);
+        const double mnd = r2d < r2pd ? r2d : r2pd;
+        const double H = MFLAG ? J.m * h2d : h2d;
+        if ((mnd < H) && (j < nj)) {
+            // Slices append in any order (pass 2 sorts); entries beyond cap
+            // only count (overflow).
+            const int k2 = atomicAdd(&nbc[i], 1);
+            if (k2 < cap) nbl[(size_t)i * cap + k2] = j;
+        } else if (r2d != 0.0) {
+            // Rare (band pairs, r2 < 1e-30): exact FP64 1/sqrt.
+            const double rinv = 1.0 / sqrt(r2d);
+            const double rinv2 = rinv * rinv;
+            const double mr = J.m * rinv;
+            const double mr3 = mr * rinv2;
+            const double rv = fma(dz, dvz, fma(dy, dvy, dx * dvx));
+            const double al = (-3.0 * rv) * rinv2;
+            Ph += mr;
+            Ax = fma(mr3, dx, Ax);
+            Ay = fma(mr3, dy, Ay);
+            Az = fma(mr3, dz, Az);
+            Jx = fma(mr3, fma(al, dx, dvx), Jx);
+            Jy = fma(mr3, fma(al, dy, dvy), Jy);
+            Jz = fma(mr3, fma(al, dz, dvz), Jz);
         }
-        qn = 0;
     };
 
     // The records of the next group (and the next tile record) are loaded
@@ -589,8 +505,7 @@ __global__ __launch_bounds__(64) void k_regf_pass1(
                     Dz = (float)(T.cz - z0);
         const float Vx = (float)(T.cu - u0), Vy = (float)(T.cv - w0),
                     Vz = (float)(T.cw - q0);
-        const bool unsafe = fmaf(adtr, T.ev, T.e) > lim;
-        const float Bt = !act ? B : (unsafe ? INFINITY : fmaxf(B, T.eb));
+        const float Bt = fmaxf(B, T.eb);
         const float d = sqrtf(fmaf(Dz, Dz, fmaf(Dy, Dy, Dx * Dx)));
         const float dvn = sqrtf(fmaf(Vz, Vz, fmaf(Vy, Vy, Vx * Vx)));
         const float R = fmaf(d, 1.0f - 1e-5f, -T.e);
@@ -599,75 +514,108 @@ __global__ __launch_bounds__(64) void k_regf_pass1(
         const bool far =
             !act || (R > fmaf(sqrtf(fmaf(A, A, Tmax)), 1.0f + 1e-5f, A));
         if (__ballot(!far) == 0) {
-            JRec32 Jn[G];
+            float ax = 0.f, ay = 0.f, az = 0.f, jx = 0.f, jy = 0.f, jz = 0.f,
+                  ph = 0.f;
+            auto pair32 = [&](const JRec32 &J) {
+                const float fx = Dx + J.ox, fy = Dy + J.oy, fz = Dz + J.oz;
+                const float gx = Vx + J.ovx, gy = Vy + J.ovy, gz = Vz + J.ovz;
+                const float r2 = fmaf(fz, fz, fmaf(fy, fy, fx * fx));
+                const float rv = fmaf(fz, gz, fmaf(fy, gy, fx * gx));
+                const float rinv = __builtin_amdgcn_rsqf(r2);
+                const float rinv2 = rinv * rinv;
+                const float mr = J.m * rinv;
+                const float mr3 = mr * rinv2;
+                const float al = (-3.0f * rv) * rinv2;
+                ph += mr;
+                ax = fmaf(mr3, fx, ax);
+                ay = fmaf(mr3, fy, ay);
+                az = fmaf(mr3, fz, az);
+                jx = fmaf(mr3, fmaf(al, fx, gx), jx);
+                jy = fmaf(mr3, fmaf(al, fy, gy), jy);
+                jz = fmaf(mr3, fmaf(al, fz, gz), jz);
+            };
+            // Two record buffers in turn: the next group's records load
+            // while the current group is evaluated, with no register copies.
+            JRec32 Pa[G], Pb[G];
 #pragma unroll
-            for (int k = 0; k < G; ++k) Jn[k] = j32[jt + k];
-            for (int jg = 0; jg < kTile; jg += G) {
-                JRec32 J[G];
-#pragma unroll
-                for (int k = 0; k < G; ++k) J[k] = Jn[k];
-#pragma unroll
-                for (int k = 0; k < G; ++k) Jn[k] = j32[jt + jg + G + k];
-#pragma unroll
-                for (int k = 0; k < G; ++k) {
-                    const float fx = Dx + J[k].ox, fy = Dy + J[k].oy,
-                                fz = Dz + J[k].oz;
-                    const float gx = Vx + J[k].ovx, gy = Vy + J[k].ovy,
-                                gz = Vz + J[k].ovz;
-                    const float r2 = fmaf(fz, fz, fmaf(fy, fy, fx * fx));
-                    const float rv = fmaf(fz, gz, fmaf(fy, gy, fx * gx));
-                    terms32(J[k].m, fx, fy, fz, gx, gy, gz, rv,
-                            __builtin_amdgcn_rsqf(r2));
-                }
+            for (int k = 0; k < G; ++k) {
+                Pa[k] = j32[jt + k];
+                opaque_sgpr(Pa[k]);
             }
-        } else {
-            JRec32 Jn[G];
+            for (int jg = 0; jg < kTile; jg += 2 * G) {
 #pragma unroll
-            for (int k = 0; k < G; ++k) Jn[k] = j32[jt + k];
-            for (int jg = 0; jg < kTile; jg += G) {
-                JRec32 J[G];
+                for (int k = 0; k < G; ++k) Pb[k] = j32[jt + jg + G + k];
+#pragma unroll
+                for (int k = 0; k < G; ++k) pair32(Pa[k]);
+#pragma unroll
+                for (int k = 0; k < G; ++k) Pa[k] = j32[jt + jg + 2 * G + k];
+#pragma unroll
+                for (int k = 0; k < G; ++k) pair32(Pb[k]);
+            }
+            Ax += (double)ax;
+            Ay += (double)ay;
+            Az += (double)az;
+            Jx += (double)jx;
+            Jy += (double)jy;
+            Jz += (double)jz;
+            Ph += (double)ph;
+        } else {
+            // FP64 group of G records at sorted position jb: the screen
+            // flags the pairs the exact predicate decides.
+            auto group64 = [&](const JRec (&J)[G], int jb) {
                 bool fl[G];
                 bool any = false;
 #pragma unroll
-                for (int k = 0; k < G; ++k) J[k] = Jn[k];
-#pragma unroll
-                for (int k = 0; k < G; ++k) Jn[k] = j32[jt + jg + G + k];
-#pragma unroll
                 for (int k = 0; k < G; ++k) {
-                    const float fx = Dx + J[k].ox, fy = Dy + J[k].oy,
-                                fz = Dz + J[k].oz;
-                    const float gx = Vx + J[k].ovx, gy = Vy + J[k].ovy,
-                                gz = Vz + J[k].ovz;
-                    const float r2 = fmaf(fz, fz, fmaf(fy, fy, fx * fx));
-                    const float rv = fmaf(fz, gz, fmaf(fy, gy, fx * gx));
-                    const float q = fmaf(dtr2x, rv, r2);
-                    fl[k] = fminf(r2, q) < Bt * J[k].w;
+                    const double dx = J[k].x - x0;
+                    const double dy = J[k].y - y0;
+                    const double dz = J[k].z - z0;
+                    const double dvx = J[k].vx - u0;
+                    const double dvy = J[k].vy - w0;
+                    const double dvz = J[k].vz - q0;
+                    const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+                    const double rv = fma(dz, dvz, fma(dy, dvy, dx * dvx));
+                    const double q = fma(dtr2x, rv, r2);
+                    double tj = thr;
+                    if (MFLAG) tj = fmax(thr * J[k].m, 1e-30);
+                    fl[k] = fmin(r2, q) < tj;
                     any = any || fl[k];
-                    const float rinv = fl[k] ? 0.0f : __builtin_amdgcn_rsqf(r2);
-                    terms32(J[k].m, fx, fy, fz, gx, gy, gz, rv, rinv);
+                    double y = __builtin_amdgcn_rsq(r2);
+                    y = fma(y, 0.5 * fma(-r2 * y, y, 1.0), y);
+                    const double rinv = fl[k] ? 0.0 : y;
+                    const double rinv2 = rinv * rinv;
+                    const double mr = J[k].m * rinv;
+                    const double mr3 = mr * rinv2;
+                    const double al = (-3.0 * rv) * rinv2;
+                    Ph += mr;
+                    Ax = fma(mr3, dx, Ax);
+                    Ay = fma(mr3, dy, Ay);
+                    Az = fma(mr3, dz, Az);
+                    Jx = fma(mr3, fma(al, dx, dvx), Jx);
+                    Jy = fma(mr3, fma(al, dy, dvy), Jy);
+                    Jz = fma(mr3, fma(al, dz, dvz), Jz);
                 }
-                if (__ballot(any) != 0) {
+                if (any) {
 #pragma unroll
-                    for (int k = 0; k < G; ++k) {
-                        if (fl[k]) {
-                            qbuf[qn][lane] = (uint16_t)(jt + jg + k - jbeg);
-                            ++qn;
-                        }
-                    }
+                    for (int k = 0; k < G; ++k)
+                        if (fl[k]) resolve(J[k], jb + k);
                 }
+            };
+            JRec Pa[G], Pb[G];
+#pragma unroll
+            for (int k = 0; k < G; ++k) {
+                Pa[k] = jps[jt + k];
+                opaque_sgpr(Pa[k]);
+            }
+            for (int jg = 0; jg < kTile; jg += 2 * G) {
+#pragma unroll
+                for (int k = 0; k < G; ++k) Pb[k] = jps[jt + jg + G + k];
+                group64(Pa, jt + jg);
+#pragma unroll
+                for (int k = 0; k < G; ++k) Pa[k] = jps[jt + jg + 2 * G + k];
+                group64(Pb, jt + jg + G);
             }
         }
-        Ax += (double)ax;
-        Ay += (double)ay;
-        Az += (double)az;
-        Jx += (double)jx;
-        Jy += (double)jy;
-        Jz += (double)jz;
-        Ph += (double)ph;
-        ax = ay = az = jx = jy = jz = ph = 0.f;
-        // A tile adds at most kTile entries per lane: flush when the next
-        // one might not fit, and after the last tile.
-        if (__ballot(qn > kQ - kTile) != 0 || t + 1 == te) flush();
     }
 
     if (act) {
@@ -749,26 +697,8 @@ __global__ __launch_bounds__(64) void k_regf_pass1_f64(
         dtr2x = 2.0 * dtrd;
         const double chk = x0 + y0 + z0 + u0 + w0 + q0 + h2d + dtrd;
         if (chk != chk) atomicOr(err, ERR_NAN_I);
-        // The FP32 screen's bound with the FP64 unit roundoff: |q_64 - q| <=
-        // 4.1u S <= 4.1u H (1 + 2 rho)^2 for the screen value, and the
-        // oracle's r2p (separate multiply and add in the prediction) within
-        // 3.1u H + 4.1u H (1 + 2 rho) of the exact one near the threshold;
-        // the threshold allows twice their sum.
         if (h2d > 0.0) {
-            const double vmax = bitsd(jst->vmax_bits);
-            double hmin = h2d;
-            if (MFLAG) {
-                const double mmin = bitsd(jst->mmin_bits);
-                hmin = (mmin < INFINITY) ? h2d * mmin : h2d;
-            }
-            const double u = 1.1102230246251565e-16;  // 2^-53
-            const double rho =
-                fabs(dtrd) * (speed_up(u0, w0, q0) + vmax) / sqrt(hmin);
-            const double g = 1.0 + 2.0 * rho;
-            const double eps = 2.0 * (8.0 * u * g * g + 4.0 * u) +
-                               (MFLAG ? 24.0 : 16.0) * u;
-            const double tt = h2d + h2d * eps;
-            thr = fmax(nextafter(nextafter(tt, INFINITY), INFINITY), 1e-30);
+            thr = f64_screen_thr<MFLAG>(h2d, dtrd, u0, w0, q0, jst);
         } else {
             thr = 1e-30;
         }
@@ -835,7 +765,10 @@ __global__ __launch_bounds__(64) void k_regf_pass1_f64(
     // so the look-ahead of the last group stays in bounds.
     JRec Jn[G];
 #pragma unroll
-    for (int k = 0; k < G; ++k) Jn[k] = jp[tb * kTile + k];
+    for (int k = 0; k < G; ++k) {
+        Jn[k] = jp[tb * kTile + k];
+        opaque_sgpr(Jn[k]);
+    }
     for (int t = tb; t < te; ++t) {
         const int jt = t * kTile;
         for (int jg = 0; jg < kTile; jg += G) {
