@@ -160,6 +160,25 @@ int gpunb_regf_dev(int ni, const double *d_h2, const double *d_dtr,
                    int *d_nnb, int *d_nnid, int *d_list, int *d_off,
                    int *d_total, long long list_cap);
 
+/* gpunb_regf_dev that allocates nothing.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, results and asynchrony as gpunb_regf_dev; its scratch (cluster
+ * table, list counts and entries, slice partials) lives in a 48 MiB device
+ * array of the library, one per GPU, held until the process ends.  A call
+ * needs (ni rounded up to 64) * (4 + 4 max(nnbmax, 1) + 64) bytes of it
+ * (fewer j slices are used when the partials of the default decomposition
+ * do not fit); a larger call is refused (-1, gpunb_last_error() says so).
+ * The i run in the caller's order (no Morton sort).  Calls share the array:
+ * two calls that overlap on one GPU race (calls on the library stream run
+ * in order). */
+int gpunb_regf_dev_noalloc(int ni, const double *d_h2, const double *d_dtr,
+                           const double *d_xi, const double *d_vi,
+                           double *d_acc, double *d_jrk, double *d_pot,
+                           int nnbmax, int m_flag, int *d_nnb, int *d_nnid,
+                           int *d_list, int *d_off, int *d_total,
+                           long long list_cap);
+
 /* ------------------------------------------------------------------------ */
 /* Ensemble of independent clusters (no reference counterpart: the reference */
 /* runs one cluster per process, -D ENSEMBLE, nbody6.F:35).  All clusters'   */

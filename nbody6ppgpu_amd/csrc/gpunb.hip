@@ -278,6 +278,15 @@ struct State {
 State S;
 bool g_abort = true;
 
+// Scratch of gpunb_regf_dev_noalloc: a file-scope device array (each GPU
+// holds its own copy until the process ends).  Host code never passes the
+// array's name to a kernel: scratch_dev[d] is its device address on GPU d,
+// taken once per device by hipGetSymbolAddress while that device is current.
+constexpr size_t kNoAllocBytes = 48ull << 20;
+constexpr int kMaxDevices = 64;
+__device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+unsigned char *scratch_dev[kMaxDevices] = {};
+
 bool g_abort_set = false;
 std::string g_last_error;
 std::mutex g_error_mu;   // entry points may fail on several threads at once
@@ -327,6 +336,12 @@ void devinit(int irank) {
     S.dev = dev;
     S.rank = irank;
     HIPCHK(hipSetDevice(dev));
+    REQUIRE(dev < kMaxDevices, "device index beyond the library's table");
+    if (!scratch_dev[dev]) {
+        void *p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        scratch_dev[dev] = (unsigned char *)p;
+    }
     HIPCHK(hipStreamCreateWithFlags(&S.st, hipStreamNonBlocking));
     HIPCHK(hipStreamCreateWithFlags(&S.st2, hipStreamNonBlocking));
     for (int k = 0; k < 2; ++k) {
@@ -437,15 +452,13 @@ int resident_waves(int m_flag, int group, int prec) {
 }
 
 // j slices per i-wavefront: enough waves to fill the resident capacity
-// `rounds` times; slices hold whole tiles, at most 1024 tiles (the FP64
-// queue holds slice-local 16-bit positions).
+// `rounds` times; slices hold whole tiles.
 int decompose(int ni_c, int ntile, int m_flag, int group, int prec) {
     const int n_iw = cdiv(ni_c, 64);
     const double target = resident_waves(m_flag, group, prec) * pick_rounds(ni_c);
     int ns = (int)std::lround(target / n_iw);
     ns = std::max(ns, 1);
     ns = std::min(ns, std::max(ntile, 1));
-    ns = std::max(ns, cdiv(ntile, 1024));
     return ns;
 }
 
@@ -642,6 +655,86 @@ void enqueue_regf(int ni, const double *xi, const double *vi, const double *h2,
         HIPCHK(hipEventRecord(S.ev_p2end, S.st2));
         HIPCHK(hipStreamWaitEvent(S.st, S.ev_p2end, 0));
     }
+}
+
+// gpunb_regf_dev without any allocation: one launch pair whose scratch (the
+// cluster table, list counts and entries, slice partials) is carved out of
+// the device array `scratch`.  The decomposition is the default one, with
+// fewer slices when the partials would not fit; a call whose lists alone
+// (or one slice of partials) do not fit is refused.  Calls share the array:
+// two calls that overlap on one GPU race.
+void enqueue_regf_noalloc(int ni, const double *xi, const double *vi,
+                          const double *h2, const double *dtr, double *acc,
+                          double *jrk, double *pot, int nnbmax, int m_flag,
+                          int *nnb, int *nnid, int *list, int *off, int *total,
+                          long long list_cap) {
+    unsigned char *base = scratch_dev[S.dev];
+    REQUIRE(base, "gpunb_regf_dev_noalloc: scratch address not initialised");
+    const int group = pick_group();
+    const int prec = pick_prec();
+    const int ntile = cdiv(S.nj, kTile);
+    const int cap = std::max(nnbmax, 1);
+    const size_t nip = (size_t)round_up(ni, 64);
+    auto a256 = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t o_nbc = a256(sizeof(ClusterDesc));
+    const size_t o_nbl = o_nbc + a256(nip * sizeof(int));
+    const size_t o_part = o_nbl + a256(nip * (size_t)cap * sizeof(int));
+    const int ns_min = 1;
+    REQUIRE(o_part + nip * ns_min * 8 * sizeof(double) <= kNoAllocBytes,
+            "gpunb_regf_dev_noalloc: ni * nnbmax too large for the 48 MiB "
+            "scratch (use gpunb_regf_dev)");
+    int ns = ntile == 0 ? 1 : decompose(ni, ntile, m_flag, group, prec);
+    const size_t fit = (kNoAllocBytes - o_part) / (nip * 8 * sizeof(double));
+    ns = (int)std::max<size_t>(ns_min, std::min<size_t>(ns, fit));
+    ClusterDesc D;
+    memset(&D, 0, sizeof D);
+    D.nj = S.nj;
+    D.ntile = ntile;
+    D.ni = ni;
+    D.n_iw = cdiv(ni, 64);
+    D.nslice = ns;
+    D.cap = cap;
+    D.ipb = pass2_ipb(ni);
+    ClusterDesc *d_desc = (ClusterDesc *)base;
+    int *nbc = (int *)(base + o_nbc);
+    launch_set_desc(S.st, d_desc, D);
+    launch_zero_int(S.st, nbc, (int)nip);
+    RegfLaunch L;
+    memset(&L, 0, sizeof L);
+    L.jps = S.sv.jps.p;
+    L.j32 = S.sv.j32.p;
+    L.tiles = S.sv.tiles.p;
+    L.jidx = S.sv.jidx.p;
+    L.jcl = S.sv.jcl.p;
+    L.jst = S.jst.p;
+    L.xi = xi;
+    L.vi = vi;
+    L.h2 = h2;
+    L.dtr = dtr;
+    L.part = (double *)(base + o_part);
+    L.nbc = nbc;
+    L.nbl = (int *)(base + o_nbl);
+    L.acc = acc;
+    L.jrk = jrk;
+    L.pot = pot;
+    L.nnb = nnb;
+    L.nnid = nnid;
+    L.list = list;
+    L.off = off;
+    L.total = total;
+    L.list_cap = list_cap;
+    L.nnbmax = nnbmax;
+    L.m_flag = m_flag;
+    L.group = group;
+    L.prec = prec;
+    L.err = S.err.p;
+    L.desc = d_desc;
+    L.ncl = 1;
+    L.ntask = (long long)D.n_iw * D.nslice;
+    L.nblk2 = cdiv(D.ni, D.ipb);
+    launch_regf_pass1(S.st, L);
+    launch_regf_pass2(S.st, L);
+    HIPCHK(hipGetLastError());
 }
 
 void require_open(const char *what) {
@@ -1212,7 +1305,7 @@ void ens_regf_dev(int ncl, const int *ni, const double *h2, const double *dtr,
         D.i0 = (int)i0;
         D.ni = ni[c];
         D.n_iw = cdiv(ni[c], 64);
-        D.nslice = std::max(1, std::max(cdiv(ntile, Lc), cdiv(ntile, 1024)));
+        D.nslice = std::max(1, cdiv(ntile, Lc));
         D.nslice = std::min(D.nslice, std::max(ntile, 1));
         D.cap = std::max(nnbmax, 1);
         D.ipb = pass2_ipb(ni[c]);
@@ -1401,6 +1494,28 @@ GPUNB_EXPORT int gpunb_regf_dev(int ni, const double *d_h2, const double *d_dtr,
         S.icall++;
         enqueue_regf(ni, d_xi, d_vi, d_h2, d_dtr, d_acc, d_jrk, d_pot, nnbmax,
                      m_flag, d_nnb, d_nnid, d_list, d_off, d_total, list_cap);
+    });
+}
+
+GPUNB_EXPORT int gpunb_regf_dev_noalloc(
+    int ni, const double *d_h2, const double *d_dtr, const double *d_xi,
+    const double *d_vi, double *d_acc, double *d_jrk, double *d_pot,
+    int nnbmax, int m_flag, int *d_nnb, int *d_nnid, int *d_list, int *d_off,
+    int *d_total, long long list_cap) {
+    return guarded("gpunb_regf_dev_noalloc", [&] {
+        require_open("gpunb_regf_dev_noalloc");
+        REQUIRE(ni >= 0 && nnbmax >= 0, "gpunb_regf_dev_noalloc: bad sizes");
+        REQUIRE(nnbmax <= kMaxNnbmax,
+                "gpunb_regf_dev_noalloc: nnbmax above 4000 is not supported");
+        set_device();
+        launch_zero_int(S.st, d_total, 1);
+        if (ni == 0) return;
+        S.numInter += (long long)ni * S.nj;
+        S.ini += ni;
+        S.icall++;
+        enqueue_regf_noalloc(ni, d_xi, d_vi, d_h2, d_dtr, d_acc, d_jrk, d_pot,
+                             nnbmax, m_flag, d_nnb, d_nnid, d_list, d_off,
+                             d_total, list_cap);
     });
 }
 

@@ -247,6 +247,8 @@ int pass2_ipb(int ni);   // pass-2 i per workgroup for a cluster of ni
 int pass2_sortcap(int nnbmax);
 constexpr int kMaxNnbmax = 4000;
 void launch_zero_int(hipStream_t st, int *p, int n);
+// *d = v, in stream order (a cluster table without a host buffer).
+void launch_set_desc(hipStream_t st, ClusterDesc *d, const ClusterDesc &v);
 // bytes (a multiple of 16, both pointers 16-byte aligned) from device memory
 // to a mapped pinned host buffer, and *err to *herr (host-mapped).
 void launch_copy_to_host(hipStream_t st, const void *src, void *dst,

@@ -298,6 +298,8 @@ __global__ __launch_bounds__(256) void k_copy_from_host(
     if (blockIdx.x == 0 && threadIdx.x == 0) *zero = 0;
 }
 
+__global__ void k_set_desc(ClusterDesc *d, ClusterDesc v) { *d = v; }
+
 __global__ void k_zero_int(int *p, int n) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n) p[k] = 0;
@@ -1140,6 +1142,10 @@ void launch_copy_from_host(hipStream_t st, const void *src, void *dst,
     const int nb = (int)std::min<size_t>(1024, std::max<size_t>(1, (n16 + 255) / 256));
     hipLaunchKernelGGL(k_copy_from_host, dim3(nb), dim3(256), 0, st,
                        (const uint4 *)src, (uint4 *)dst, n16, zero);
+}
+
+void launch_set_desc(hipStream_t st, ClusterDesc *d, const ClusterDesc &v) {
+    hipLaunchKernelGGL(k_set_desc, dim3(1), dim3(1), 0, st, d, v);
 }
 
 void launch_zero_int(hipStream_t st, int *p, int n) {

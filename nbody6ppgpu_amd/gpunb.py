@@ -115,6 +115,11 @@ def _declare(lib: ctypes.CDLL) -> None:
                            [ctypes.c_int] + [ctypes.c_void_p] * 7 +
                            [ctypes.c_int, ctypes.c_int] +
                            [ctypes.c_void_p] * 5 + [ctypes.c_longlong]),
+        "gpunb_regf_dev_noalloc": (ctypes.c_int,
+                                   [ctypes.c_int] + [ctypes.c_void_p] * 7 +
+                                   [ctypes.c_int, ctypes.c_int] +
+                                   [ctypes.c_void_p] * 5 +
+                                   [ctypes.c_longlong]),
         "gpunb_regcor_dev": (ctypes.c_int,
                              [ctypes.POINTER(RegcorPar),
                               ctypes.POINTER(RegcorState), ctypes.c_int] +
