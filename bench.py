@@ -692,12 +692,15 @@ def main():
     # |dE/E| over one N-body time unit: the host Hermite / Ahmad-Cohen driver
     # (host/hermite_ac.cpp) integrates the same cluster through the engine's
     # ABI (device predictor + gpunb_regf_ + gpupot_) with ETAI 0.05 and
-    # ETAR 0.1, once with the mixed-precision pair kernel (FP64 differences,
-    # FP32 pair arithmetic) and once with the all-FP64 kernel, side by side.
+    # ETAR 0.1, once with the library's default kernel choice (the mixed
+    # kernel for blocks of at least 8192 i, run in Morton order; the FP64
+    # kernel for smaller, scattered blocks) and once with the FP64 kernel
+    # for every block, side by side.
     hermite = []
     if not args.no_hermite:
         from nbody6ppgpu_amd import integrate
-        for prec, label in ((32, "mixed FP32/FP64"),
+        for prec, label in ((0, "default: mixed FP32/FP64 for Morton-sorted "
+                                "blocks >= 8192 i, FP64 below"),
                             (65, "FP64 (v_rsq_f64 + one Newton step)")):
             log(f"bench: Hermite/AC integration, ETAI 0.05 ETAR 0.1, {label}")
             rec = integrate.run(c.m, c.x, c.v, t_end=1.0, etai=0.05, etar=0.1,

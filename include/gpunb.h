@@ -314,9 +314,12 @@ const char *gpunb_version(void);
 int gpunb_set_tuning(int group, double rounds);
 
 /* Pair arithmetic of the regular-force kernel: 32 = FP32 pair arithmetic on
- * FP64 differences (default), 64 = FP64 pair arithmetic with the hardware
- * 1/sqrt, 65 = FP64 with one Newton step on 1/sqrt; 0 = default.  Neighbour
- * lists are identical in every mode.  Returns 0 or -1. */
+ * tile-relative differences, with the tiles near an i evaluated in FP64;
+ * 64 = FP64 pair arithmetic with the hardware 1/sqrt; 65 = FP64 with one
+ * Newton step on 1/sqrt; 0 = default: 32 for calls whose i are run in
+ * Morton order (gpunb_regf_ / gpunb_regf_dev with ni >= 8192), 65 for the
+ * others (scattered i leave few tiles far from every i of a wavefront).
+ * Neighbour lists are identical in every mode.  Returns 0 or -1. */
 int gpunb_set_precision(int prec);
 
 /* Kernel timing: with enable != 0 the library records HIP events on its own

@@ -416,10 +416,9 @@ int pick_group() {
     const char *g = getenv("GPUNB_REGF_GROUP");
     if (g && atoi(g) == 2) return 2;
     if (g && atoi(g) == 4) return 4;
-    // Two pairs per group: 90 VGPRs, 5 resident waves per SIMD (four pairs:
-    // more than 128 VGPRs, 3 waves), which hides the latency of the
-    // dependent FP32 / FP64 chains better; measured 3.26 ms against 4.07 ms
-    // per 4.3e9-pair launch on MI355X.
+    // Two pairs per group: the mixed kernel fits 4 resident waves per SIMD
+    // and the FP64 kernel 5 (four pairs: more than 128 VGPRs, 3 waves),
+    // which hides the latency of the dependent chains better.
     return 2;
 }
 
@@ -437,11 +436,17 @@ double pick_rounds(int ni) {
     return std::min(6.0, std::max(0.5, ni / 1024.0));
 }
 
-int pick_prec() {
+// Pair arithmetic of a call.  Default: the mixed kernel (32) when the i run
+// in Morton order, since its FP32 far tiles need waves of nearby i; the
+// FP64 kernel with one Newton step (65) otherwise: with i scattered over the
+// cluster nearly every tile is near some lane, and the mixed kernel would
+// run its FP64 loop almost everywhere with fewer resident waves (King
+// N = 131072, 1024 scattered i: pass 1 220 us mixed).
+int pick_prec(bool i_sorted) {
     if (S.prec) return S.prec;
     const char *p = getenv("GPUNB_PREC");
     if (p && (atoi(p) == 32 || atoi(p) == 64 || atoi(p) == 65)) return atoi(p);
-    return 32;
+    return i_sorted ? 32 : 65;
 }
 
 int resident_waves(int m_flag, int group, int prec) {
@@ -451,14 +456,15 @@ int resident_waves(int m_flag, int group, int prec) {
     return r;
 }
 
-// j slices per i-wavefront: enough waves to fill the resident capacity
-// `rounds` times; slices hold whole tiles.
+// j slices per i-wavefront: enough wavefronts (kP1W per slice task) to
+// fill the resident capacity `rounds` times; a slice holds at least one
+// tile per wavefront where the j set allows.
 int decompose(int ni_c, int ntile, int m_flag, int group, int prec) {
     const int n_iw = cdiv(ni_c, 64);
     const double target = resident_waves(m_flag, group, prec) * pick_rounds(ni_c);
-    int ns = (int)std::lround(target / n_iw);
+    int ns = (int)std::lround(target / ((double)n_iw * kP1W));
     ns = std::max(ns, 1);
-    ns = std::min(ns, std::max(ntile, 1));
+    ns = std::min(ns, std::max(ntile / kP1W, 1));
     return ns;
 }
 
@@ -515,12 +521,13 @@ void enqueue_regf(int ni, const double *xi, const double *vi, const double *h2,
                   int nnbmax, int m_flag, int *nnb, int *nnid, int *list,
                   int *off, int *total, long long list_cap) {
     const int group = pick_group();
-    const int prec = pick_prec();
+    const bool isort = S.nj > 0 && pick_isort(ni);
+    const int prec = pick_prec(isort);
     const int ntile = cdiv(S.nj, kTile);
     const int chunk = pick_chunk();
     const int nch = cdiv(ni, chunk);
     const int *perm = nullptr;
-    if (S.nj > 0 && pick_isort(ni)) {
+    if (isort) {
         const size_t n = (size_t)ni;
         S.ikey_in.ensure(n);
         S.ikey_out.ensure(n);
@@ -671,7 +678,7 @@ void enqueue_regf_noalloc(int ni, const double *xi, const double *vi,
     unsigned char *base = scratch_dev[S.dev];
     REQUIRE(base, "gpunb_regf_dev_noalloc: scratch address not initialised");
     const int group = pick_group();
-    const int prec = pick_prec();
+    const int prec = pick_prec(false);
     const int ntile = cdiv(S.nj, kTile);
     const int cap = std::max(nnbmax, 1);
     const size_t nip = (size_t)round_up(ni, 64);
@@ -1247,7 +1254,8 @@ int ens_slice_tiles(const int *ni, int group, int prec, int m_flag) {
     auto waves = [&](int L) {
         double w = 0;
         for (int c = 0; c < E.ncl; ++c)
-            w += (double)cdiv(ni[c], 64) * std::max(1, cdiv(cdiv(E.nj[c], kTile), L));
+            w += (double)cdiv(ni[c], 64) * kP1W *
+                 std::max(1, cdiv(cdiv(E.nj[c], kTile), L));
         return w;
     };
     while (lo < hi) {               // smallest L with waves(L) <= target
@@ -1268,7 +1276,7 @@ void ens_regf_dev(int ncl, const int *ni, const double *h2, const double *dtr,
     REQUIRE(nnbmax <= kMaxNnbmax, "gpunb_ens_regf_dev: nnbmax above 4000 is not supported (a list row is sorted in LDS; the reference LMAX is at most 600)");
     set_device();
     const int group = pick_group();
-    const int prec = pick_prec();
+    const int prec = pick_prec(false);
     const int L = ens_slice_tiles(ni, group, prec, m_flag);
     // The launch key: a graph is replayed only for identical launches.
     std::vector<long long> key = {ncl, nnbmax, m_flag, group, prec, L,

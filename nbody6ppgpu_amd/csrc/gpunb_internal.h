@@ -69,6 +69,11 @@ struct JClus {
     double pad;
 };
 
+// Wavefronts per pass-1 workgroup: they split one j slice of one
+// i-wavefront and reduce their sums in LDS, so a slice partial covers
+// kP1W times the tiles one wavefront runs.
+constexpr int kP1W = 4;
+
 // Near band of the FP32 pair loop: a tile holding a pair with min(r2, q) <
 // K_BAND * h2 * w_j (times <m> with mass-weighted lists) for some lane of a
 // wave is evaluated in FP64 by that wave.
@@ -165,7 +170,8 @@ struct RegfLaunch {
     // decomposition
     int nslice;      // j slices per i-wave
     // pass-1 scratch
-    double *part;    // [ni_pad][nslice][8] slice partials
+    double *part;    // [nslice][ni_pad][8] slice partials (slice-major: a
+                     // pass-1 workgroup writes 64 consecutive records)
     int *nbc;        // [ni] list counts: zero before pass 1, reset by pass 2
     int *nbl;        // [ni][cap] list entries (sorted positions)
     // i in sorted order: output row of the i at (call-wide) position p is

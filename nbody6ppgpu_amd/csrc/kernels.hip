@@ -305,6 +305,31 @@ __global__ void k_zero_int(int *p, int n) {
     if (k < n) p[k] = 0;
 }
 
+// The kP1W wavefronts of a pass-1 workgroup share one (i-wavefront, j
+// slice) task, each over a contiguous part of the slice's tiles; their FP64
+// sums are added in wavefront order through LDS and the first wavefront
+// writes the slice partial of its lanes (a fixed order: the results do not
+// depend on which wavefront finishes first).
+__device__ inline void slice_partial_store(const double (&v)[7], int wv,
+                                           int lane, bool act, double *p) {
+    __shared__ double red[kP1W - 1][7][64];
+    if (wv > 0) {
+#pragma unroll
+        for (int c = 0; c < 7; ++c) red[wv - 1][c][lane] = v[c];
+    }
+    __syncthreads();
+    if (wv == 0 && act) {
+#pragma unroll
+        for (int c = 0; c < 7; ++c) {
+            double t = v[c];
+#pragma unroll
+            for (int w = 0; w < kP1W - 1; ++w) t += red[w][c][lane];
+            p[c] = t;
+        }
+        p[7] = 0.0;
+    }
+}
+
 // Threshold of the FP64 screen min(r2, r2 + 2 dtr dx.dv) < thr (times m_j
 // with mass-weighted lists) for a lane with h2 > 0: pairs below it are
 // decided by the oracle's exact predicate.  The FP32-form screen's value in
@@ -388,7 +413,7 @@ __device__ inline void opaque_sgpr(JRec &r) {
 }
 
 template <bool MFLAG, int G>
-__global__ __launch_bounds__(64) void k_regf_pass1(
+__global__ __launch_bounds__(64 * kP1W) void k_regf_pass1(
     const JRec32 *__restrict__ j32_all, const JTile *__restrict__ tile_all,
     const JRec *__restrict__ jps_all, const JClus *__restrict__ jcl_all,
     const ClusterDesc *__restrict__ desc, int ncl, long long ntask,
@@ -420,12 +445,18 @@ __global__ __launch_bounds__(64) void k_regf_pass1(
     const int task = (int)(gtask - D.task0);
     const int s = task / n_iw;
     const int iw = task - s * n_iw;
-    const int lane = threadIdx.x;
+    // Wavefront index (uniform within the wavefront: keeps the j record
+    // addresses scalar).
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
     const int i = iw * 64 + lane;
     const bool act = i < ni;
 
-    const int tb = (int)(((long long)ntile * s) / nslice);
-    const int te = (int)(((long long)ntile * (s + 1)) / nslice);
+    // Slice s of the j tiles, and this wavefront's contiguous share of it.
+    const int sb = (int)(((long long)ntile * s) / nslice);
+    const int se = (int)(((long long)ntile * (s + 1)) / nslice);
+    const int tb = sb + (int)(((long long)(se - sb) * wv) / kP1W);
+    const int te = sb + (int)(((long long)(se - sb) * (wv + 1)) / kP1W);
 
     double x0 = 0.0, y0 = 0.0, z0 = 0.0, u0 = 0.0, w0 = 0.0, q0 = 0.0;
     double h2d = 0.0, dtrd = 0.0, dtr2x = 0.0;   // dtr2x = 2 dtr
@@ -620,17 +651,9 @@ __global__ __launch_bounds__(64) void k_regf_pass1(
         }
     }
 
-    if (act) {
-        double *p = part + ((size_t)i * nslice + s) * 8;
-        p[0] = Ax;
-        p[1] = Ay;
-        p[2] = Az;
-        p[3] = Jx;
-        p[4] = Jy;
-        p[5] = Jz;
-        p[6] = Ph;
-        p[7] = 0.0;
-    }
+    double sums[7] = {Ax, Ay, Az, Jx, Jy, Jz, Ph};
+    slice_partial_store(sums, wv, lane, act,
+                        part + ((size_t)s * n_iw * 64 + i) * 8);
 }
 
 // ---------------------------------------------------------------------------
@@ -650,7 +673,7 @@ __global__ __launch_bounds__(64) void k_regf_pass1(
 // ---------------------------------------------------------------------------
 
 template <bool MFLAG, int G, int NR>
-__global__ __launch_bounds__(64) void k_regf_pass1_f64(
+__global__ __launch_bounds__(64 * kP1W) void k_regf_pass1_f64(
     const JRec *__restrict__ jp_all, const ClusterDesc *__restrict__ desc,
     int ncl, long long ntask, const JStats *__restrict__ jst,
     const double *__restrict__ xi_all, const double *__restrict__ vi_all,
@@ -677,12 +700,18 @@ __global__ __launch_bounds__(64) void k_regf_pass1_f64(
     const int task = (int)(gtask - D.task0);
     const int s = task / n_iw;
     const int iw = task - s * n_iw;
-    const int lane = threadIdx.x;
+    // Wavefront index (uniform within the wavefront: keeps the j record
+    // addresses scalar).
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
     const int i = iw * 64 + lane;
     const bool act = i < ni;
 
-    const int tb = (int)(((long long)ntile * s) / nslice);
-    const int te = (int)(((long long)ntile * (s + 1)) / nslice);
+    // Slice s of the j tiles, and this wavefront's contiguous share of it.
+    const int sb = (int)(((long long)ntile * s) / nslice);
+    const int se = (int)(((long long)ntile * (s + 1)) / nslice);
+    const int tb = sb + (int)(((long long)(se - sb) * wv) / kP1W);
+    const int te = sb + (int)(((long long)(se - sb) * (wv + 1)) / kP1W);
 
     double x0 = 0.0, y0 = 0.0, z0 = 0.0, u0 = 0.0, w0 = 0.0, q0 = 0.0;
     double h2d = 0.0, dtrd = 0.0, dtr2x = 0.0;   // dtr2x = 2 dtr
@@ -807,17 +836,10 @@ __global__ __launch_bounds__(64) void k_regf_pass1_f64(
         }
     }
 
-    if (act) {
-        double *p = part + ((size_t)i * nslice + s) * 8;
-        p[0] = ax;
-        p[1] = ay;
-        p[2] = az;
-        p[3] = fma(-3.0, j2x, j1x);
-        p[4] = fma(-3.0, j2y, j1y);
-        p[5] = fma(-3.0, j2z, j1z);
-        p[6] = ph;
-        p[7] = 0.0;
-    }
+    double sums[7] = {ax, ay, az, fma(-3.0, j2x, j1x), fma(-3.0, j2y, j1y),
+                      fma(-3.0, j2z, j1z), ph};
+    slice_partial_store(sums, wv, lane, act,
+                        part + ((size_t)s * n_iw * 64 + i) * 8);
 }
 
 // ---------------------------------------------------------------------------
@@ -884,7 +906,8 @@ __global__ __launch_bounds__(256) void k_regf_pass2(
                 for (int u = 0; u < kP2U; ++u) {
                     const int q = b0 + 64 * u + lane;
                     if (q < nslice) {
-                        const double *p = part + ((size_t)i * nslice + q) * 8;
+                        const double *p =
+                            part + ((size_t)q * D.n_iw * 64 + i) * 8;
 #pragma unroll
                         for (int k = 0; k < 7; ++k) v[u][k] = p[k];
                     } else {
@@ -1155,7 +1178,8 @@ void launch_zero_int(hipStream_t st, int *p, int n) {
 
 template <bool MFLAG, int G>
 static void launch_pass1(hipStream_t st, const RegfLaunch &L, long long nblk) {
-    hipLaunchKernelGGL((k_regf_pass1<MFLAG, G>), dim3((unsigned)nblk), dim3(64),
+    hipLaunchKernelGGL((k_regf_pass1<MFLAG, G>), dim3((unsigned)nblk),
+                       dim3(64 * kP1W),
                        0, st, L.j32, L.tiles, L.jps, L.jcl, L.desc, L.ncl,
                        L.ntask, L.jst, L.xi, L.vi, L.h2, L.dtr, L.part, L.nbc,
                        L.nbl, L.err);
@@ -1165,7 +1189,7 @@ template <bool MFLAG, int G, int NR>
 static void launch_pass1_f64(hipStream_t st, const RegfLaunch &L,
                              long long nblk) {
     hipLaunchKernelGGL((k_regf_pass1_f64<MFLAG, G, NR>), dim3((unsigned)nblk),
-                       dim3(64), 0, st, L.jps, L.desc, L.ncl, L.ntask, L.jst,
+                       dim3(64 * kP1W), 0, st, L.jps, L.desc, L.ncl, L.ntask, L.jst,
                        L.xi, L.vi, L.h2, L.dtr, L.part, L.nbc, L.nbl, L.err);
 }
 
@@ -1236,31 +1260,32 @@ void launch_pot(hipStream_t st, const PRec *p, int n, int ntile, int i0,
                        ni, nslice, pot);
 }
 
-static int resident_waves_of(const void *fn) {
+// Resident wavefronts of a kernel launched in blocks of `block` threads.
+static int resident_waves_of(const void *fn, int block = 64) {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 0;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return 0;
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 64, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, block, 0) !=
         hipSuccess)
         return 0;
-    return nb * prop.multiProcessorCount;
+    return nb * (block / 64) * prop.multiProcessorCount;
 }
 
 int regf_resident_waves(int m_flag, int group, int prec) {
     if (prec >= 64) {
         if (group == 2)
-            return m_flag ? resident_waves_of((const void *)k_regf_pass1_f64<true, 2, 1>)
-                          : resident_waves_of((const void *)k_regf_pass1_f64<false, 2, 1>);
-        return m_flag ? resident_waves_of((const void *)k_regf_pass1_f64<true, 4, 1>)
-                      : resident_waves_of((const void *)k_regf_pass1_f64<false, 4, 1>);
+            return m_flag ? resident_waves_of((const void *)k_regf_pass1_f64<true, 2, 1>, 64 * kP1W)
+                          : resident_waves_of((const void *)k_regf_pass1_f64<false, 2, 1>, 64 * kP1W);
+        return m_flag ? resident_waves_of((const void *)k_regf_pass1_f64<true, 4, 1>, 64 * kP1W)
+                      : resident_waves_of((const void *)k_regf_pass1_f64<false, 4, 1>, 64 * kP1W);
     }
     if (group == 2)
-        return m_flag ? resident_waves_of((const void *)k_regf_pass1<true, 2>)
-                      : resident_waves_of((const void *)k_regf_pass1<false, 2>);
-    return m_flag ? resident_waves_of((const void *)k_regf_pass1<true, 4>)
-                  : resident_waves_of((const void *)k_regf_pass1<false, 4>);
+        return m_flag ? resident_waves_of((const void *)k_regf_pass1<true, 2>, 64 * kP1W)
+                      : resident_waves_of((const void *)k_regf_pass1<false, 2>, 64 * kP1W);
+    return m_flag ? resident_waves_of((const void *)k_regf_pass1<true, 4>, 64 * kP1W)
+                  : resident_waves_of((const void *)k_regf_pass1<false, 4>, 64 * kP1W);
 }
 
 int pot_resident_waves() { return resident_waves_of((const void *)k_pot_pass1); }
