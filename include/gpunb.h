@@ -210,6 +210,14 @@ int gpunb_ens_regf_dev(int ncl, const int *ni, const double *d_h2,
  * replay it while sizes and pointers stay the same (re-captured otherwise). */
 int gpunb_ens_graph(int enable);
 
+/* enable != 0 (the default): pass 1 of the ensemble launch runs as a
+ * persistent grid of resident workgroups that take (cluster, i-wavefront,
+ * j slice) tasks from per-XCD atomic queues, largest tasks first, with the
+ * task list cut into XCD ranges of equal work (a workgroup whose range is
+ * used up takes tasks from the next ranges); 0: one workgroup per task.
+ * Results are identical either way.  Returns 0 or -1. */
+int gpunb_ens_persistent(int enable);
+
 /* Autotuner output: j tiles (64 j) per slice for clusters whose j count lies
  * in [2^b, 2^(b+1)), b < nbucket; 0 = automatic.  Returns 0 or -1. */
 int gpunb_ens_set_bucket_tiles(int nbucket, const int *tiles);

@@ -540,7 +540,7 @@ void enqueue_regf(int ni, const double *xi, const double *vi, const double *h2,
             S.itmp.ensure(need);
             S.itmp_bytes = need;
         }
-        ISortBufs B;
+        ISortBufs B = {};
         B.key_in = S.ikey_in.p;
         B.key_out = S.ikey_out.p;
         B.val_in = S.ival_in.p;
@@ -1134,6 +1134,20 @@ struct Ens {
     hipGraphExec_t gexec = nullptr;
     std::vector<long long> gkey;
     long long last_pairs = 0;
+    // The i of every cluster in Morton order (for the mixed kernel): keys,
+    // permutation, gathered i data, and the clusters' i starts.
+    DevBuf<unsigned long long> ikey_in, ikey_out;
+    DevBuf<int> ival_in, iperm, d_i0;
+    DevBuf<char> itmp;
+    size_t itmp_bytes = 0;
+    DevBuf<double> isorted;
+    HostBuf<int> h_i0;
+    std::vector<int> i0_up;   // the starts last uploaded to d_i0
+    // Persistent pass 1: task queues per XCD and their task ranges.
+    bool persistent = true;
+    DevBuf<unsigned> queue;
+    DevBuf<long long> d_xr;
+    HostBuf<long long> h_xr;
 };
 Ens E;
 
@@ -1160,6 +1174,19 @@ void ens_close() {
     E.part.release();
     E.nbc.release();
     E.nbl.release();
+    E.ikey_in.release();
+    E.ikey_out.release();
+    E.ival_in.release();
+    E.iperm.release();
+    E.d_i0.release();
+    E.itmp.release();
+    E.itmp_bytes = 0;
+    E.isorted.release();
+    E.h_i0.release();
+    E.i0_up.clear();
+    E.queue.release();
+    E.d_xr.release();
+    E.h_xr.release();
     E.nbc_dirty = false;
     E.open = false;
     E.ncl = 0;
@@ -1276,7 +1303,60 @@ void ens_regf_dev(int ncl, const int *ni, const double *h2, const double *dtr,
     REQUIRE(nnbmax <= kMaxNnbmax, "gpunb_ens_regf_dev: nnbmax above 4000 is not supported (a list row is sorted in LDS; the reference LMAX is at most 600)");
     set_device();
     const int group = pick_group();
-    const int prec = pick_prec(false);
+    // Every cluster's i run in Morton order (sorted here, before the graph
+    // replay below, so a replay reads this call's order): the mixed kernel
+    // by default.
+    const int prec = pick_prec(true);
+    const int *perm = nullptr;
+    long long ni_all = 0;
+    for (int c = 0; c < ncl; ++c) {
+        REQUIRE(ni[c] >= 0, "gpunb_ens_regf_dev: ni must be >= 0");
+        ni_all += ni[c];
+    }
+    REQUIRE(ni_all < (1LL << 31), "gpunb_ens_regf_dev: too many i");
+    if (prec < 64 && ni_all > 0) {
+        std::vector<int> i0v((size_t)ncl + 1, 0);
+        for (int c = 0; c < ncl; ++c) i0v[c + 1] = i0v[c] + ni[c];
+        if (i0v != E.i0_up) {
+            HIPCHK(hipStreamSynchronize(S.st));   // h_i0 may be in flight
+            E.h_i0.ensure((size_t)ncl + 1);
+            E.d_i0.ensure((size_t)ncl + 1);
+            memcpy(E.h_i0.p, i0v.data(), ((size_t)ncl + 1) * sizeof(int));
+            HIPCHK(hipMemcpyAsync(E.d_i0.p, E.h_i0.p, ((size_t)ncl + 1) * sizeof(int),
+                                  hipMemcpyHostToDevice, S.st));
+            E.i0_up = i0v;
+        }
+        const size_t n = (size_t)ni_all;
+        E.ikey_in.ensure(n);
+        E.ikey_out.ensure(n);
+        E.ival_in.ensure(n);
+        E.iperm.ensure(n);
+        E.isorted.ensure(8 * n);
+        const size_t need = isort_ens_tmp_bytes((int)n, ncl);
+        if (need == 0) throw Error("ensemble i sort: storage query failed");
+        if (need > E.itmp_bytes) {
+            E.itmp.ensure(need);
+            E.itmp_bytes = need;
+        }
+        ISortBufs B = {};
+        B.key64_in = E.ikey_in.p;
+        B.key64_out = E.ikey_out.p;
+        B.val_in = E.ival_in.p;
+        B.perm = E.iperm.p;
+        B.tmp = E.itmp.p;
+        B.tmp_bytes = E.itmp_bytes;
+        B.xs = E.isorted.p;
+        B.vs = B.xs + 3 * n;
+        B.hs = B.vs + 3 * n;
+        B.ds = B.hs + n;
+        HIPCHK(launch_isort_ens(S.st, E.sv.jcl.p, E.d_i0.p, ncl, xi, vi, h2, dtr,
+                                (int)n, B));
+        xi = B.xs;
+        vi = B.vs;
+        h2 = B.hs;
+        dtr = B.ds;
+        perm = B.perm;
+    }
     const int L = ens_slice_tiles(ni, group, prec, m_flag);
     // The launch key: a graph is replayed only for identical launches.
     std::vector<long long> key = {ncl, nnbmax, m_flag, group, prec, L,
@@ -1285,7 +1365,8 @@ void ens_regf_dev(int ncl, const int *ni, const double *h2, const double *dtr,
                                   (long long)acc, (long long)jrk,
                                   (long long)pot, (long long)nnb,
                                   (long long)nnid, (long long)list,
-                                  (long long)off, (long long)total, list_cap};
+                                  (long long)off, (long long)total, list_cap,
+                                  (long long)perm, (long long)E.persistent};
     for (int c = 0; c < ncl; ++c) {
         key.push_back(ni[c]);
         key.push_back(E.nj[c]);
@@ -1295,14 +1376,14 @@ void ens_regf_dev(int ncl, const int *ni, const double *h2, const double *dtr,
         HIPCHK(hipGraphLaunch(E.gexec, S.st));
         return;
     }
-    // Cluster table.
+    // Cluster table, one entry per cluster; scratch offsets in cluster order.
     HIPCHK(hipStreamSynchronize(S.st));   // the table may be in use
     E.h_desc.ensure((size_t)ncl);
     E.d_desc.ensure((size_t)ncl);
-    long long task = 0, pairs = 0, blk = 0, i0 = 0, work = 0;
+    std::vector<ClusterDesc> cd((size_t)ncl);
+    long long pairs = 0, i0 = 0, work = 0;
     for (int c = 0; c < ncl; ++c) {
-        REQUIRE(ni[c] >= 0, "gpunb_ens_regf_dev: ni must be >= 0");
-        ClusterDesc &D = E.h_desc.p[c];
+        ClusterDesc &D = cd[c];
         memset(&D, 0, sizeof D);
         const int ntile = cdiv(E.nj[c], kTile);
         const int b = std::min(23, std::max(0, (int)std::log2(std::max(E.nj[c], 1))));
@@ -1319,15 +1400,56 @@ void ens_regf_dev(int ncl, const int *ni, const double *h2, const double *dtr,
         D.ipb = pass2_ipb(ni[c]);
         D.jc = c;
         D.ibase = (int)i0;
-        D.task0 = task;
         D.pbase = pairs;
         D.sbase = i0 * D.cap;
-        D.blk0 = blk;
-        task += (long long)D.n_iw * D.nslice;
         pairs += (long long)round_up(std::max(ni[c], 1), 64) * D.nslice;
-        blk += cdiv(ni[c], D.ipb);
         i0 += ni[c];
         work += (long long)ni[c] * E.nj[c];
+    }
+    // Table order = task order.  Persistent launch: clusters by decreasing
+    // tiles per task (the largest tasks are taken first, small ones even out
+    // the end), and the task list cut into 8 ranges of equal tile count, one
+    // per XCD queue.  Plain launch: cluster order.
+    std::vector<int> order((size_t)ncl);
+    for (int c = 0; c < ncl; ++c) order[c] = c;
+    auto tiles_per_task = [&](int c) {
+        return (double)cd[c].ntile / std::max(cd[c].nslice, 1);
+    };
+    const bool persist = E.persistent;
+    if (persist)
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+            return tiles_per_task(a) > tiles_per_task(b);
+        });
+    long long task = 0, blk = 0;
+    double cost_all = 0.0;
+    for (int k = 0; k < ncl; ++k) {
+        ClusterDesc &D = cd[order[k]];
+        D.task0 = task;
+        D.blk0 = blk;
+        task += (long long)D.n_iw * D.nslice;
+        blk += cdiv(D.ni, D.ipb);
+        cost_all += (double)D.n_iw * D.nslice * tiles_per_task(order[k]);
+        E.h_desc.p[k] = D;
+    }
+    long long xr[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    {
+        // xr[x] = first task whose preceding tasks hold x/8 of the tiles.
+        double cum = 0.0;
+        int x = 1;
+        for (int k = 0; k < ncl && x < 8; ++k) {
+            const ClusterDesc &D = cd[order[k]];
+            const long long nt = (long long)D.n_iw * D.nslice;
+            const double w = tiles_per_task(order[k]);
+            while (x < 8 && nt > 0 && cum + nt * w >= cost_all * x / 8.0) {
+                const long long m = std::min<long long>(
+                    nt, std::max<long long>(
+                            0, (long long)std::ceil((cost_all * x / 8.0 - cum) / w)));
+                xr[x++] = D.task0 + m;
+            }
+            cum += nt * w;
+        }
+        for (; x < 8; ++x) xr[x] = task;
+        xr[8] = task;
     }
     E.last_pairs = work;
     E.part.ensure((size_t)pairs * 8);
@@ -1339,6 +1461,14 @@ void ens_regf_dev(int ncl, const int *ni, const double *h2, const double *dtr,
     }
     HIPCHK(hipMemcpyAsync(E.d_desc.p, E.h_desc.p, ncl * sizeof(ClusterDesc),
                           hipMemcpyHostToDevice, S.st));
+    if (persist) {
+        E.queue.ensure(8);
+        E.d_xr.ensure(9);
+        E.h_xr.ensure(9);
+        memcpy(E.h_xr.p, xr, sizeof xr);
+        HIPCHK(hipMemcpyAsync(E.d_xr.p, E.h_xr.p, sizeof xr,
+                              hipMemcpyHostToDevice, S.st));
+    }
     HIPCHK(hipStreamSynchronize(S.st));
     RegfLaunch R;
     memset(&R, 0, sizeof R);
@@ -1373,6 +1503,17 @@ void ens_regf_dev(int ncl, const int *ni, const double *h2, const double *dtr,
     R.ncl = ncl;
     R.ntask = task;
     R.nblk2 = blk;
+    R.perm = perm;
+    if (persist) {
+        // One workgroup per resident slot, a multiple of 8 (one share per
+        // XCD), no more than the tasks need.
+        const long long slots = resident_waves(m_flag, group, prec) / kP1W;
+        long long g = std::min<long long>(slots, 8 * ((task + 7) / 8));
+        g = std::max<long long>(8, g / 8 * 8);
+        R.queue = E.queue.p;
+        R.xrange = E.d_xr.p;
+        R.pgrid = (int)g;
+    }
     auto enqueue = [&]() {
         E.nbc_dirty = true;
         launch_zero_int(S.st, total, 1);
@@ -1761,6 +1902,17 @@ GPUNB_EXPORT int gpunb_ens_graph(int enable) {
         }
         ens_graph_drop();
         E.graph_on = enable != 0;
+    });
+}
+
+GPUNB_EXPORT int gpunb_ens_persistent(int enable) {
+    return guarded("gpunb_ens_persistent", [&] {
+        if (S.devinit) {
+            set_device();
+            HIPCHK(hipStreamSynchronize(S.st));
+        }
+        ens_graph_drop();
+        E.persistent = enable != 0;
     });
 }
 

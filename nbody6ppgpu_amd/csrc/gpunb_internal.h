@@ -191,6 +191,12 @@ struct RegfLaunch {
     int ncl;
     long long ntask;   // pass-1 tasks over all clusters
     long long nblk2;   // pass-2 workgroups over all clusters
+    // Persistent pass 1 (queue != NULL): pgrid workgroups take tasks from
+    // 8 per-XCD counters (cleared by the launch); XCD x owns the tasks
+    // [xrange[x], xrange[x + 1]).
+    unsigned *queue;
+    const long long *xrange;
+    int pgrid;
 };
 
 // Kernel launchers (kernels.hip).  All asynchronous on `st`.
@@ -229,7 +235,8 @@ hipError_t launch_jprep(hipStream_t st, const JView &V, const PrepClus *pc,
 // Morton order of the i of one call (jprep.hip): perm[p] = the original i
 // at sorted position p, and the i data gathered into that order.
 struct ISortBufs {
-    unsigned *key_in, *key_out;
+    unsigned *key_in, *key_out;                 // one cluster (32-bit keys)
+    unsigned long long *key64_in, *key64_out;   // ensemble (cluster << 32 | code)
     int *val_in, *perm;
     void *tmp;
     size_t tmp_bytes;
@@ -239,6 +246,14 @@ size_t isort_tmp_bytes(int n);
 hipError_t launch_isort(hipStream_t st, const JClus *jcl, const double *xi,
                         const double *vi, const double *h2, const double *dtr,
                         int ni, const ISortBufs &B);
+// Ensemble: the i of every cluster in Morton order within the cluster (its
+// range [cl_i0[c], cl_i0[c + 1]) of the concatenated i kept), keyed in the
+// box of the cluster's j set.
+size_t isort_ens_tmp_bytes(int n, int ncl);
+hipError_t launch_isort_ens(hipStream_t st, const JClus *jcl, const int *cl_i0,
+                            int ncl, const double *xi, const double *vi,
+                            const double *h2, const double *dtr, int ni,
+                            const ISortBufs &B);
 // *pc = v, in stream order.
 void launch_set_prepclus(hipStream_t st, PrepClus *pc, const PrepClus &v);
 // Padding records in j32[0, n) and empty tiles in tiles[0, ntile).

@@ -336,6 +336,60 @@ __global__ __launch_bounds__(256) void k_igather(
     ds[p] = dtr[i];
 }
 
+// Ensemble i keys: (cluster << 32) | Morton code of x_i in the box of the
+// cluster owning global i (the last c with cl_i0[c] <= i; clusters hold
+// consecutive i ranges in cluster order, empty ones repeat the next start).
+__global__ __launch_bounds__(256) void k_ikey_ens(
+    const double *__restrict__ xi, int ni, const int *__restrict__ cl_i0,
+    int ncl, const JClus *__restrict__ jcl, unsigned long long *__restrict__ key,
+    int *__restrict__ val) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ni) return;
+    int lo = 0, hi = ncl - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (cl_i0[mid] <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    const JClus &J = jcl[lo];
+    const double ext = fmax(fmax(J.hi[0] - J.lo[0], J.hi[1] - J.lo[1]),
+                            J.hi[2] - J.lo[2]);
+    const double scale = ext > 0.0 ? 1024.0 / ext : 0.0;
+    const unsigned code =
+        spread3(cell10(xi[3 * (size_t)g + 0], J.lo[0], scale)) |
+        (spread3(cell10(xi[3 * (size_t)g + 1], J.lo[1], scale)) << 1) |
+        (spread3(cell10(xi[3 * (size_t)g + 2], J.lo[2], scale)) << 2);
+    key[g] = ((unsigned long long)lo << 32) | code;
+    val[g] = g;
+}
+
+size_t isort_ens_tmp_bytes(int n, int ncl) {
+    size_t bytes = 0;
+    const hipError_t rc = rocprim::radix_sort_pairs(
+        nullptr, bytes, (const unsigned long long *)nullptr,
+        (unsigned long long *)nullptr, (const int *)nullptr, (int *)nullptr,
+        (size_t)std::max(n, 1), 0u, key_bits(ncl), (hipStream_t)0);
+    return rc == hipSuccess ? bytes : 0;
+}
+
+hipError_t launch_isort_ens(hipStream_t st, const JClus *jcl, const int *cl_i0,
+                            int ncl, const double *xi, const double *vi,
+                            const double *h2, const double *dtr, int ni,
+                            const ISortBufs &B) {
+    if (ni <= 0) return hipSuccess;
+    const dim3 g((ni + 255) / 256);
+    hipLaunchKernelGGL(k_ikey_ens, g, dim3(256), 0, st, xi, ni, cl_i0, ncl, jcl,
+                       B.key64_in, B.val_in);
+    size_t bytes = B.tmp_bytes;
+    const hipError_t rc = rocprim::radix_sort_pairs(
+        B.tmp, bytes, B.key64_in, B.key64_out, B.val_in, B.perm, (size_t)ni, 0u,
+        key_bits(ncl), st);
+    if (rc != hipSuccess) return rc;
+    hipLaunchKernelGGL(k_igather, g, dim3(256), 0, st, B.perm, ni, xi, vi, h2,
+                       dtr, B.xs, B.vs, B.hs, B.ds);
+    return hipGetLastError();
+}
+
 size_t isort_tmp_bytes(int n) {
     size_t bytes = 0;
     const hipError_t rc = rocprim::radix_sort_pairs(

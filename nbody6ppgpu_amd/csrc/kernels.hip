@@ -412,8 +412,11 @@ __device__ inline void opaque_sgpr(JRec &r) {
                       "+s"(r.vy), "+s"(r.vz), "+s"(r.m));
 }
 
+// One pass-1 task of the mixed kernel (a workgroup: one i-wavefront x one j
+// slice, task number gtask < ntask).
 template <bool MFLAG, int G>
-__global__ __launch_bounds__(64 * kP1W) void k_regf_pass1(
+__device__ __forceinline__ void regf_pass1_task(
+    long long gtask,
     const JRec32 *__restrict__ j32_all, const JTile *__restrict__ tile_all,
     const JRec *__restrict__ jps_all, const JClus *__restrict__ jcl_all,
     const ClusterDesc *__restrict__ desc, int ncl, long long ntask,
@@ -422,12 +425,6 @@ __global__ __launch_bounds__(64 * kP1W) void k_regf_pass1(
     const double *__restrict__ dtr_all, double *__restrict__ part_all,
     int *__restrict__ nbc_all, int *__restrict__ nbl_all,
     unsigned *__restrict__ err) {
-    // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
-    // contiguous range of tasks so that it streams only its share of the j
-    // array through its own L2.  Tasks are slice-major within a cluster.
-    const long long per = (ntask + 7) >> 3;
-    const long long gtask = (long long)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
-    if (gtask >= ntask) return;
     const ClusterDesc &D = desc[find_task(desc, ncl, gtask)];
     const int nj = D.nj, ntile = D.ntile, ni = D.ni, n_iw = D.n_iw;
     const int nslice = D.nslice, cap = D.cap;
@@ -672,20 +669,16 @@ __global__ __launch_bounds__(64 * kP1W) void k_regf_pass1(
 // combined as J1 - 3 J2 at the end (one multiply per pair less).
 // ---------------------------------------------------------------------------
 
+// One pass-1 task of the FP64 kernel.
 template <bool MFLAG, int G, int NR>
-__global__ __launch_bounds__(64 * kP1W) void k_regf_pass1_f64(
+__device__ __forceinline__ void regf_pass1_f64_task(
+    long long gtask,
     const JRec *__restrict__ jp_all, const ClusterDesc *__restrict__ desc,
     int ncl, long long ntask, const JStats *__restrict__ jst,
     const double *__restrict__ xi_all, const double *__restrict__ vi_all,
     const double *__restrict__ h2_all, const double *__restrict__ dtr_all,
     double *__restrict__ part_all, int *__restrict__ nbc_all,
     int *__restrict__ nbl_all, unsigned *__restrict__ err) {
-    // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
-    // contiguous range of tasks so that it streams only its share of the j
-    // array through its own L2.  Tasks are slice-major within a cluster.
-    const long long per = (ntask + 7) >> 3;
-    const long long gtask = (long long)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
-    if (gtask >= ntask) return;
     const ClusterDesc &D = desc[find_task(desc, ncl, gtask)];
     const int nj = D.nj, ntile = D.ntile, ni = D.ni, n_iw = D.n_iw;
     const int nslice = D.nslice, cap = D.cap;
@@ -840,6 +833,117 @@ __global__ __launch_bounds__(64 * kP1W) void k_regf_pass1_f64(
                       fma(-3.0, j2z, j1z), ph};
     slice_partial_store(sums, wv, lane, act,
                         part + ((size_t)s * n_iw * 64 + i) * 8);
+}
+
+
+// Block-to-task map of a plain launch: workgroups are dispatched round-robin
+// over the 8 XCDs, so each XCD is given a contiguous range of tasks and
+// streams only its share of the j array through its own L2 (tasks are
+// slice-major within a cluster).  -1: no task for this workgroup.
+__device__ inline long long xcd_task(long long ntask) {
+    const long long per = (ntask + 7) >> 3;
+    const long long t = (long long)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    return t < ntask ? t : -1;
+}
+
+// Next task of a persistent workgroup.  Tasks [xr[x], xr[x + 1]) belong to
+// XCD x (the workgroup's XCD is blockIdx.x & 7: workgroups of a grid that
+// fits the machine are dispatched round-robin over the XCDs) and are taken in
+// order through the counter q[x]; once its own range is used up a workgroup
+// takes tasks from the following XCDs' ranges.  Returns -1 when every range
+// is used up, so every workgroup ends.  Thread 0 takes the task; x and tries
+// are its state across calls.
+__device__ inline long long next_task(unsigned *__restrict__ q,
+                                      const long long *__restrict__ xr, int &x,
+                                      int &tries) {
+    __shared__ long long s_task;
+    __syncthreads();   // every thread has read the previous task
+    if (threadIdx.x == 0) {
+        long long t = -1;
+        while (tries < 8) {
+            const long long lo = xr[x], hi = xr[x + 1];
+            if (lo < hi) {
+                const long long k = (long long)atomicAdd(&q[x], 1u);
+                if (lo + k < hi) {
+                    t = lo + k;
+                    break;
+                }
+            }
+            x = (x + 1) & 7;
+            ++tries;
+        }
+        s_task = t;
+    }
+    __syncthreads();
+    return s_task;
+}
+
+template <bool MFLAG, int G>
+__global__ __launch_bounds__(64 * kP1W) void k_regf_pass1(
+    const JRec32 *__restrict__ j32_all, const JTile *__restrict__ tile_all,
+    const JRec *__restrict__ jps_all, const JClus *__restrict__ jcl_all,
+    const ClusterDesc *__restrict__ desc, int ncl, long long ntask,
+    const JStats *__restrict__ jst, const double *__restrict__ xi_all,
+    const double *__restrict__ vi_all, const double *__restrict__ h2_all,
+    const double *__restrict__ dtr_all, double *__restrict__ part_all,
+    int *__restrict__ nbc_all, int *__restrict__ nbl_all,
+    unsigned *__restrict__ err) {
+    const long long t = xcd_task(ntask);
+    if (t < 0) return;
+    regf_pass1_task<MFLAG, G>(t, j32_all, tile_all, jps_all, jcl_all, desc, ncl, ntask, jst, xi_all, vi_all,
+        h2_all, dtr_all, part_all, nbc_all, nbl_all, err);
+}
+
+// Persistent form: a grid of resident workgroups takes tasks from the XCD
+// queues until all are done (the ensemble's uneven tasks, largest first).
+template <bool MFLAG, int G>
+__global__ __launch_bounds__(64 * kP1W) void k_regf_pass1_q(
+    const JRec32 *__restrict__ j32_all, const JTile *__restrict__ tile_all,
+    const JRec *__restrict__ jps_all, const JClus *__restrict__ jcl_all,
+    const ClusterDesc *__restrict__ desc, int ncl, long long ntask,
+    const JStats *__restrict__ jst, const double *__restrict__ xi_all,
+    const double *__restrict__ vi_all, const double *__restrict__ h2_all,
+    const double *__restrict__ dtr_all, double *__restrict__ part_all,
+    int *__restrict__ nbc_all, int *__restrict__ nbl_all,
+    unsigned *__restrict__ err, unsigned *__restrict__ q, const long long *__restrict__ xr) {
+    int x = blockIdx.x & 7, tries = 0;
+    for (;;) {
+        const long long t = next_task(q, xr, x, tries);
+        if (t < 0) return;
+        regf_pass1_task<MFLAG, G>(t, j32_all, tile_all, jps_all, jcl_all, desc, ncl, ntask, jst, xi_all, vi_all,
+        h2_all, dtr_all, part_all, nbc_all, nbl_all, err);
+    }
+}
+
+template <bool MFLAG, int G, int NR>
+__global__ __launch_bounds__(64 * kP1W) void k_regf_pass1_f64(
+    const JRec *__restrict__ jp_all, const ClusterDesc *__restrict__ desc,
+    int ncl, long long ntask, const JStats *__restrict__ jst,
+    const double *__restrict__ xi_all, const double *__restrict__ vi_all,
+    const double *__restrict__ h2_all, const double *__restrict__ dtr_all,
+    double *__restrict__ part_all, int *__restrict__ nbc_all,
+    int *__restrict__ nbl_all, unsigned *__restrict__ err) {
+    const long long t = xcd_task(ntask);
+    if (t < 0) return;
+    regf_pass1_f64_task<MFLAG, G, NR>(t, jp_all, desc, ncl, ntask, jst, xi_all, vi_all, h2_all, dtr_all,
+        part_all, nbc_all, nbl_all, err);
+}
+
+template <bool MFLAG, int G, int NR>
+__global__ __launch_bounds__(64 * kP1W) void k_regf_pass1_f64_q(
+    const JRec *__restrict__ jp_all, const ClusterDesc *__restrict__ desc,
+    int ncl, long long ntask, const JStats *__restrict__ jst,
+    const double *__restrict__ xi_all, const double *__restrict__ vi_all,
+    const double *__restrict__ h2_all, const double *__restrict__ dtr_all,
+    double *__restrict__ part_all, int *__restrict__ nbc_all,
+    int *__restrict__ nbl_all, unsigned *__restrict__ err, unsigned *__restrict__ q, const long long *__restrict__ xr) {
+    int x = blockIdx.x & 7, tries = 0;
+    for (;;) {
+        const long long t = next_task(q, xr, x, tries);
+        if (t < 0) return;
+        regf_pass1_f64_task<MFLAG, G, NR>(t, jp_all, desc, ncl, ntask, jst, xi_all, vi_all, h2_all, dtr_all,
+        part_all, nbc_all, nbl_all, err);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1178,6 +1282,14 @@ void launch_zero_int(hipStream_t st, int *p, int n) {
 
 template <bool MFLAG, int G>
 static void launch_pass1(hipStream_t st, const RegfLaunch &L, long long nblk) {
+    if (L.queue) {
+        hipLaunchKernelGGL((k_regf_pass1_q<MFLAG, G>), dim3((unsigned)L.pgrid),
+                           dim3(64 * kP1W), 0, st, L.j32, L.tiles, L.jps, L.jcl,
+                           L.desc, L.ncl, L.ntask, L.jst, L.xi, L.vi, L.h2,
+                           L.dtr, L.part, L.nbc, L.nbl, L.err, L.queue,
+                           L.xrange);
+        return;
+    }
     hipLaunchKernelGGL((k_regf_pass1<MFLAG, G>), dim3((unsigned)nblk),
                        dim3(64 * kP1W),
                        0, st, L.j32, L.tiles, L.jps, L.jcl, L.desc, L.ncl,
@@ -1188,6 +1300,14 @@ static void launch_pass1(hipStream_t st, const RegfLaunch &L, long long nblk) {
 template <bool MFLAG, int G, int NR>
 static void launch_pass1_f64(hipStream_t st, const RegfLaunch &L,
                              long long nblk) {
+    if (L.queue) {
+        hipLaunchKernelGGL((k_regf_pass1_f64_q<MFLAG, G, NR>),
+                           dim3((unsigned)L.pgrid), dim3(64 * kP1W), 0, st,
+                           L.jps, L.desc, L.ncl, L.ntask, L.jst, L.xi, L.vi,
+                           L.h2, L.dtr, L.part, L.nbc, L.nbl, L.err, L.queue,
+                           L.xrange);
+        return;
+    }
     hipLaunchKernelGGL((k_regf_pass1_f64<MFLAG, G, NR>), dim3((unsigned)nblk),
                        dim3(64 * kP1W), 0, st, L.jps, L.desc, L.ncl, L.ntask, L.jst,
                        L.xi, L.vi, L.h2, L.dtr, L.part, L.nbc, L.nbl, L.err);
@@ -1196,6 +1316,7 @@ static void launch_pass1_f64(hipStream_t st, const RegfLaunch &L,
 void launch_regf_pass1(hipStream_t st, const RegfLaunch &L) {
     if (L.ntask <= 0) return;
     const long long nblk = 8 * ((L.ntask + 7) / 8);
+    if (L.queue) launch_zero_int(st, (int *)L.queue, 8);
     if (L.prec >= 64) {
         const int nr = L.prec - 64;   // 64: no Newton step, 65: one step
         if (L.group == 2) {

@@ -142,6 +142,7 @@ def _declare(lib: ctypes.CDLL) -> None:
                                [ctypes.c_int, ctypes.c_int] +
                                [ctypes.c_void_p] * 5 + [ctypes.c_longlong]),
         "gpunb_ens_graph": (ctypes.c_int, [ctypes.c_int]),
+        "gpunb_ens_persistent": (ctypes.c_int, [ctypes.c_int]),
         "gpunb_ens_set_bucket_tiles": (ctypes.c_int, [ctypes.c_int, _ip]),
         "gpunb_ens_pairs": (ctypes.c_longlong, []),
         "gpunb_mlp_nparams": (ctypes.c_int, []),
@@ -348,6 +349,11 @@ def gpunb_ens_send(clusters) -> None:
 
 def gpunb_ens_graph(enable: bool) -> None:
     if load().gpunb_ens_graph(1 if enable else 0) != 0:
+        _check()
+
+
+def gpunb_ens_persistent(enable: bool) -> None:
+    if load().gpunb_ens_persistent(1 if enable else 0) != 0:
         _check()
 
 

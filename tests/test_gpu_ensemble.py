@@ -156,3 +156,45 @@ def test_bucket_tiles_change_decomposition_not_results():
     finally:
         gpunb.gpunb_ens_set_bucket_tiles([])
         gpunb.gpunb_ens_close()
+
+
+@pytest.mark.parametrize("prec", [0, 65])
+def test_persistent_launch_matches_plain(prec):
+    """The persistent pass 1 (per-XCD task queues, largest tasks first)
+    computes every (i-wavefront, slice) partial exactly as the one-workgroup-
+    per-task launch does: results bit-identical, in the default (mixed,
+    Morton-sorted i) and the FP64 kernel."""
+    cl = make_clusters([6000, 300, 2500, 64, 12000, 900], 801)
+    nnbmax = 300
+    ni = [c[0].n for c in cl]
+    ni[3] = 17
+    gpunb.gpunb_ens_open([c[0].n for c in cl])
+    try:
+        gpunb.gpunb_set_precision(prec)
+        gpunb.gpunb_ens_send([(c[0].m, c[0].x, c[0].v) for c in cl])
+        run = EnsRun(cl, ni, nnbmax)
+        gpunb.gpunb_ens_persistent(False)
+        run.call()
+        plain = run.results()
+        gpunb.gpunb_ens_persistent(True)
+        for rep in range(2):
+            run.acc.zero()
+            run.nnb.zero()
+            run.call()
+            got = run.results()
+            for k, (a, b) in enumerate(zip(got, plain)):
+                for u, w in zip(a, b):
+                    assert np.array_equal(u, w), (prec, rep, k)
+    finally:
+        gpunb.gpunb_ens_persistent(True)
+        gpunb.gpunb_set_precision(0)
+        gpunb.gpunb_ens_close()
+    o = cl[4]
+    c, h2, dtr = o
+    sub = np.arange(0, c.n, 97)
+    r = plain[4]
+    g = (r[0][sub], r[1][sub], r[2][sub], r[3][sub], r[4][sub])
+    ref = oracle.regf(c.m, c.x, c.v, h2[sub], dtr[sub], c.x[sub], c.v[sub],
+                      nnbmax + 1, nnbmax, 0)
+    assert_parity(g, ref, label=f"persistent ensemble cluster 4 (prec {prec})",
+                  inputs=(c.m, c.x, c.v, c.x[sub], c.v[sub]))
