@@ -200,6 +200,10 @@ struct State {
     int nbmax = 0;
     int nj = 0;       // current j count used by regf
     int njp = 0;      // j slots covered by the predictor state
+    // The sorted view of the j set is rebuilt lazily: send and predict mark
+    // it stale, and only calls that run the mixed kernel or sort their i
+    // rebuild it (the FP64 kernel reads the caller-order records).
+    bool sv_stale = false;
     DevBuf<JRec> jp;
     SortedView sv;
     DevBuf<JStats> jst;
@@ -514,6 +518,32 @@ void ensure_regf_scratch(int ni_c, int nslice, int cap, int set) {
 // Enqueue the regular-force kernels for device-resident i data.
 void wait_event(hipEvent_t ev, bool &live);
 
+// The sorted view of the current j set, rebuilt if a send or predict made it
+// stale.
+void ensure_view() {
+    if (!S.sv_stale) return;
+    S.sv.prepare_one(S.st, S.jp.p, S.nj);
+    S.sv_stale = false;
+}
+
+// j side of a launch: the sorted view for the mixed kernel, the
+// caller-order records (identity position map) for the FP64 kernel.
+void set_j_side(RegfLaunch &L, int prec) {
+    if (prec < 64) {
+        L.jps = S.sv.jps.p;
+        L.j32 = S.sv.j32.p;
+        L.tiles = S.sv.tiles.p;
+        L.jidx = S.sv.jidx.p;
+        L.jcl = S.sv.jcl.p;
+    } else {
+        L.jps = S.jp.p;
+        L.j32 = nullptr;
+        L.tiles = nullptr;
+        L.jidx = nullptr;
+        L.jcl = nullptr;
+    }
+}
+
 // Enqueue the regular-force kernels for device-resident i data: one launch
 // pair per chunk of pick_chunk() i, each a one-cluster table.
 void enqueue_regf(int ni, const double *xi, const double *vi, const double *h2,
@@ -523,6 +553,7 @@ void enqueue_regf(int ni, const double *xi, const double *vi, const double *h2,
     const int group = pick_group();
     const bool isort = S.nj > 0 && pick_isort(ni);
     const int prec = pick_prec(isort);
+    if (isort || prec < 64) ensure_view();
     const int ntile = cdiv(S.nj, kTile);
     const int chunk = pick_chunk();
     const int nch = cdiv(ni, chunk);
@@ -595,11 +626,7 @@ void enqueue_regf(int ni, const double *xi, const double *vi, const double *h2,
         const ClusterDesc &D = nd[c];
         RegfLaunch L;
         memset(&L, 0, sizeof L);
-        L.jps = S.sv.jps.p;
-        L.j32 = S.sv.j32.p;
-        L.tiles = S.sv.tiles.p;
-        L.jidx = S.sv.jidx.p;
-        L.jcl = S.sv.jcl.p;
+        set_j_side(L, prec);
         L.jst = S.jst.p;
         L.xi = xi;
         L.vi = vi;
@@ -708,11 +735,8 @@ void enqueue_regf_noalloc(int ni, const double *xi, const double *vi,
     launch_zero_int(S.st, nbc, (int)nip);
     RegfLaunch L;
     memset(&L, 0, sizeof L);
-    L.jps = S.sv.jps.p;
-    L.j32 = S.sv.j32.p;
-    L.tiles = S.sv.tiles.p;
-    L.jidx = S.sv.jidx.p;
-    L.jcl = S.sv.jcl.p;
+    if (prec < 64) ensure_view();
+    set_j_side(L, prec);
     L.jst = S.jst.p;
     L.xi = xi;
     L.vi = vi;
@@ -878,7 +902,7 @@ void do_send(int nj, const double *m, const double (*x)[3],
     launch_stats_reset(S.st, S.jst.p, nullptr);
     launch_pack_j(S.st, S.d_send.p, S.d_send.p + nj, S.d_send.p + 4 * (size_t)nj,
                   nj, pad, S.jp.p, S.jst.p, S.err.p);
-    S.sv.prepare_one(S.st, S.jp.p, nj);
+    S.sv_stale = true;
     HIPCHK(hipGetLastError());
     S.nj = nj;
     S.time_send += wtime() - t0;
@@ -1084,7 +1108,7 @@ void do_predict(double t) {
     launch_stats_reset(S.st, S.jst.p, nullptr);
     launch_predict(S.st, t, S.njp, pad, S.x0.p, S.v0.p, S.f.p, S.fd.p, S.t0.p,
                    S.mass.p, S.jp.p, S.jst.p);
-    S.sv.prepare_one(S.st, S.jp.p, S.njp);
+    S.sv_stale = true;
     HIPCHK(hipGetLastError());
     S.nj = S.njp;
 }

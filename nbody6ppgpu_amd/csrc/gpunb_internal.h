@@ -159,7 +159,7 @@ struct RegfLaunch {
     const JRec *jps;
     const JRec32 *j32;
     const JTile *tiles;
-    const int *jidx;
+    const int *jidx;     // NULL: records in the caller's order
     const JClus *jcl;
     int nj;          // real j count
     int ntile;       // padded j count / kTile

@@ -985,7 +985,8 @@ __global__ __launch_bounds__(256) void k_regf_pass2(
     const int ipb = D.ipb;
     const int i0 = (int)(blockIdx.x - D.blk0) * ipb;
     const JRec *__restrict__ jps = jps_all + D.j0;
-    const int *__restrict__ jidx = jidx_all + D.j0;
+    // NULL: the j records are in the caller's order (position = index).
+    const int *__restrict__ jidx = jidx_all ? jidx_all + D.j0 : nullptr;
     const double *__restrict__ xi = xi_all + 3 * (size_t)D.i0;
     const double *__restrict__ part = part_all + 8 * D.pbase;
     int *__restrict__ nbc = nbc_all + D.ibase;
@@ -1081,7 +1082,7 @@ __global__ __launch_bounds__(256) void k_regf_pass2(
             int bj = INT_MAX;
             for (int e = lane; e < n; e += 64) {
                 const int p = ent[e];
-                const int j = jidx[p];
+                const int j = jidx ? jidx[p] : p;
                 row[e] = j;
                 const JRec &J = jps[p];
                 const double r2 = sq3d(J.x - xa, J.y - ya, J.z - za);
