@@ -816,11 +816,12 @@ def main():
             "precision": ("FP64 pair arithmetic (hardware v_rsq_f64, %d "
                           "Newton steps)" % (args.prec - 64))
                          if args.prec >= 64 else
-                         ("FP32 pair arithmetic on tile-relative FP32 "
-                          "differences (j Morton-sorted in 64-j tiles with "
-                          "FP64 centres); FP64 for the near band (12 h2, "
-                          "mass-weighted), the neighbour predicate and the "
-                          "accumulation"),
+                         ("FP32 pair arithmetic on tile-relative "
+                          "differences in far tiles (j Morton-sorted in 64-j "
+                          "tiles with FP64 centres, i Morton-sorted); FP64 "
+                          "pair arithmetic for every tile within a "
+                          "wavefront's near band (12 h2 w_j), the neighbour "
+                          "predicate and the accumulation"),
         },
         "roofline": {
             "bound": "valu",

@@ -515,7 +515,7 @@ void ensure_regf_scratch(int ni_c, int nslice, int cap, int set) {
     }
 }
 
-// Enqueue the regular-force kernels for device-resident i data.
+// Waits for a recorded event (defined below).
 void wait_event(hipEvent_t ev, bool &live);
 
 // The sorted view of the current j set, rebuilt if a send or predict made it

@@ -190,3 +190,195 @@ void model_regf(int nj, const double *m, const double *xj, const double *vj,
     stats[0] = s_band;
     stats[1] = s_nb;
 }
+
+/*
+ * model_regf_tiles -- the engine's mixed kernel as shipped: j in tile order
+ * (the caller passes them light-Morton then heavy-Morton, cut into tiles of
+ * 64), i in wavefronts of 64 (the caller passes them in Morton order; a
+ * partial last wavefront has inactive lanes).  Per (wavefront, tile) every
+ * lane evaluates the far test in FP32 as the kernel does
+ *   D = fl32(C_t - x_i), R = |D| (1 - 1e-5) - e_t,
+ *   A = |dtr| (1 + 2u) (|V| + ev_t) (1 + 1e-5),
+ *   far: R > sqrt(A^2 + max(B, eb_t) wmax_t (1 + 1e-5)) (1 + 1e-5) + A,
+ *   B = fl32(K h2 (<m>) (1 + 8u)) (1 + 2u);
+ * a tile that is near for any lane is evaluated in FP64 by every lane (the
+ * oracle's arithmetic, neighbours and r2 == 0 excluded), a far tile in FP32
+ * with tile-relative differences, its sums folded into FP64 at its end.
+ * stats[0] = near (wavefront, tile) pairs, stats[1] = all such pairs.
+ */
+void model_regf_tiles(int nj, const double *m, const double *xj,
+                      const double *vj, int ni, const double *h2,
+                      const double *dtr, const double *xi, const double *vi,
+                      int m_flag, double K, double *acc, double *jrk,
+                      double *pot, long long *stats) {
+    const float u32 = 5.9604645e-08f;
+    double mbar = 0.0;
+    for (int j = 0; j < nj; j++) mbar += m[j];
+    mbar = nj > 0 ? mbar / nj : 1.0;
+    const int ntile = (nj + 63) / 64;
+    /* per tile: C[6], e, ev, wmax, eb (floats kept as doubles) */
+    double *T = malloc(sizeof(double) * 10 * (size_t)(ntile > 0 ? ntile : 1));
+    float *o = malloc(sizeof(float) * 6 * (size_t)(nj > 0 ? nj : 1));
+    for (int t = 0; t < ntile; t++) {
+        double lo[6], hi[6];
+        for (int c = 0; c < 6; c++) {
+            lo[c] = INFINITY;
+            hi[c] = -INFINITY;
+        }
+        for (int j = 64 * t; j < 64 * t + 64 && j < nj; j++)
+            for (int c = 0; c < 6; c++) {
+                double v = c < 3 ? xj[3 * (size_t)j + c] : vj[3 * (size_t)j + c - 3];
+                if (v < lo[c]) lo[c] = v;
+                if (v > hi[c]) hi[c] = v;
+            }
+        double *C = T + 10 * t;
+        for (int c = 0; c < 6; c++) C[c] = 0.5 * lo[c] + 0.5 * hi[c];
+        double e = 0.0, ev = 0.0, wm = 1.0;
+        for (int j = 64 * t; j < 64 * t + 64 && j < nj; j++) {
+            double a = 0.0, b = 0.0;
+            for (int c = 0; c < 3; c++) {
+                o[6 * (size_t)j + c] = (float)(xj[3 * (size_t)j + c] - C[c]);
+                o[6 * (size_t)j + 3 + c] = (float)(vj[3 * (size_t)j + c] - C[3 + c]);
+                a += (double)o[6 * (size_t)j + c] * o[6 * (size_t)j + c];
+                b += (double)o[6 * (size_t)j + 3 + c] * o[6 * (size_t)j + 3 + c];
+            }
+            a = sqrt(a) * (1.0 + 1e-12);
+            b = sqrt(b) * (1.0 + 1e-12);
+            if (a > e) e = a;
+            if (b > ev) ev = b;
+            double w = 1.0;
+            const double x = m[j] / mbar;
+            if (mbar > 0.0 && m[j] > mbar) {
+                w = fmax(fmax(1.0, cbrt(x * x)), 1.4 * x / K) * (1.0 + 1e-12);
+                float wf = (float)w;
+                if ((double)wf < w) wf = nextafterf(wf, INFINITY);
+                w = wf;
+            }
+            if (w > wm) wm = w;
+        }
+        float ef = (float)e, evf = (float)ev;
+        if ((double)ef < e) ef = nextafterf(ef, INFINITY);
+        if ((double)evf < ev) evf = nextafterf(evf, INFINITY);
+        C[6] = ef;
+        C[7] = evf;
+        C[8] = (float)wm;
+        double eb = 0.25 * (double)ef * ef * (1.0 + 1e-12);
+        float ebf = (float)eb;
+        if ((double)ebf < eb) ebf = nextafterf(ebf, INFINITY);
+        C[9] = ebf;
+    }
+    const int nw = (ni + 63) / 64;
+    long long s_near = 0, s_all = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : s_near, s_all)
+    for (int w = 0; w < nw; w++) {
+        const int i0 = 64 * w, nl = ni - i0 < 64 ? ni - i0 : 64;
+        double A[64][7];
+        float Bl[64], adtr[64];
+        for (int l = 0; l < nl; l++) {
+            for (int c = 0; c < 7; c++) A[l][c] = 0.0;
+            const int i = i0 + l;
+            const float d32 = (float)dtr[i];
+            adtr[l] = fabsf(d32) * (1.0f + 2.0f * u32);
+            Bl[l] = 0.0f;
+            if (h2[i] > 0.0) {
+                const double bb = K * h2[i] * (m_flag ? mbar : 1.0) *
+                                  (1.0 + 8.0 * (double)u32);
+                Bl[l] = (float)bb * (1.0f + 2.0f * u32);
+            }
+        }
+        for (int t = 0; t < ntile; t++) {
+            const double *C = T + 10 * t;
+            int near = 0;
+            for (int l = 0; l < nl && !near; l++) {
+                const int i = i0 + l;
+                const float Dx = (float)(C[0] - xi[3 * (size_t)i + 0]);
+                const float Dy = (float)(C[1] - xi[3 * (size_t)i + 1]);
+                const float Dz = (float)(C[2] - xi[3 * (size_t)i + 2]);
+                const float Vx = (float)(C[3] - vi[3 * (size_t)i + 0]);
+                const float Vy = (float)(C[4] - vi[3 * (size_t)i + 1]);
+                const float Vz = (float)(C[5] - vi[3 * (size_t)i + 2]);
+                const float Bt = Bl[l] > (float)C[9] ? Bl[l] : (float)C[9];
+                const float d = sqrtf(fmaf(Dz, Dz, fmaf(Dy, Dy, Dx * Dx)));
+                const float dvn = sqrtf(fmaf(Vz, Vz, fmaf(Vy, Vy, Vx * Vx)));
+                const float R = fmaf(d, 1.0f - 1e-5f, -(float)C[6]);
+                const float Aa = adtr[l] * (dvn + (float)C[7]) * (1.0f + 1e-5f);
+                const float Tm = Bt * (float)C[8] * (1.0f + 1e-5f);
+                const int far =
+                    R > fmaf(sqrtf(fmaf(Aa, Aa, Tm)), 1.0f + 1e-5f, Aa);
+                if (!far) near = 1;
+            }
+            s_all++;
+            s_near += near;
+            const int j1 = 64 * t + 64 < nj ? 64 * t + 64 : nj;
+            for (int l = 0; l < nl; l++) {
+                const int i = i0 + l;
+                const double *x0 = xi + 3 * (size_t)i, *v0 = vi + 3 * (size_t)i;
+                float ax = 0, ay = 0, az = 0, jx = 0, jy = 0, jz = 0, ph = 0;
+                float F[6];
+                for (int c = 0; c < 3; c++) {
+                    F[c] = (float)(C[c] - x0[c]);
+                    F[3 + c] = (float)(C[3 + c] - v0[c]);
+                }
+                for (int j = 64 * t; j < j1; j++) {
+                    if (near) {
+                        const double *X = xj + 3 * (size_t)j, *V = vj + 3 * (size_t)j;
+                        double dd[6];
+                        for (int c = 0; c < 3; c++) {
+                            dd[c] = X[c] - x0[c];
+                            dd[3 + c] = V[c] - v0[c];
+                        }
+                        const double r2d = sq3(dd[0], dd[1], dd[2]);
+                        const double r2pd = sq3(dd[0] + dtr[i] * dd[3],
+                                                dd[1] + dtr[i] * dd[4],
+                                                dd[2] + dtr[i] * dd[5]);
+                        const double mn = r2d < r2pd ? r2d : r2pd;
+                        const double H = m_flag ? m[j] * h2[i] : h2[i];
+                        if (mn < H || r2d == 0.0) continue;
+                        const double rinv = 1.0 / sqrt(r2d);
+                        const double rinv2 = rinv * rinv;
+                        const double mr = m[j] * rinv, mr3 = mr * rinv2;
+                        const double rvd = dd[0] * dd[3] + dd[1] * dd[4] + dd[2] * dd[5];
+                        const double al = -3.0 * rvd * rinv2;
+                        A[l][6] += mr;
+                        for (int c = 0; c < 3; c++) {
+                            A[l][c] += mr3 * dd[c];
+                            A[l][3 + c] += mr3 * (dd[3 + c] + al * dd[c]);
+                        }
+                    } else {
+                        const float *oj = o + 6 * (size_t)j;
+                        float f[6];
+                        for (int c = 0; c < 6; c++) f[c] = F[c] + oj[c];
+                        const float r2 = fmaf(f[2], f[2], fmaf(f[1], f[1], f[0] * f[0]));
+                        const float rv = fmaf(f[2], f[5], fmaf(f[1], f[4], f[0] * f[3]));
+                        const float rinv = rsq_model(r2, (uint32_t)i, (uint32_t)j);
+                        const float rinv2 = rinv * rinv;
+                        const float mr = (float)m[j] * rinv;
+                        const float mr3 = mr * rinv2;
+                        const float al = (-3.0f * rv) * rinv2;
+                        ph += mr;
+                        ax = fmaf(mr3, f[0], ax);
+                        ay = fmaf(mr3, f[1], ay);
+                        az = fmaf(mr3, f[2], az);
+                        jx = fmaf(mr3, fmaf(al, f[0], f[3]), jx);
+                        jy = fmaf(mr3, fmaf(al, f[1], f[4]), jy);
+                        jz = fmaf(mr3, fmaf(al, f[2], f[5]), jz);
+                    }
+                }
+                A[l][0] += ax; A[l][1] += ay; A[l][2] += az;
+                A[l][3] += jx; A[l][4] += jy; A[l][5] += jz; A[l][6] += ph;
+            }
+        }
+        for (int l = 0; l < nl; l++) {
+            const int i = i0 + l;
+            for (int c = 0; c < 3; c++) {
+                acc[3 * (size_t)i + c] = A[l][c];
+                jrk[3 * (size_t)i + c] = A[l][3 + c];
+            }
+            pot[i] = A[l][6];
+        }
+    }
+    free(T);
+    free(o);
+    stats[0] = s_near;
+    stats[1] = s_all;
+}

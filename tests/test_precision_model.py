@@ -31,6 +31,7 @@ from oracle.parity import FORCE_TOL
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 BAND = 12.0
+HEAVY = 2.8       # bodies above HEAVY <m> are tiled after the light ones
 _dp = ctypes.POINTER(ctypes.c_double)
 
 
@@ -41,6 +42,11 @@ def model(tmp_path_factory):
                            "-shared", "-fPIC", "-o", so,
                            os.path.join(HERE, "fp32_model.c"), "-lm"])
     lib = ctypes.CDLL(so)
+    lib.model_regf_tiles.restype = None
+    lib.model_regf_tiles.argtypes = ([ctypes.c_int, _dp, _dp, _dp,
+                                      ctypes.c_int, _dp, _dp, _dp, _dp,
+                                      ctypes.c_int, ctypes.c_double, _dp, _dp,
+                                      _dp, ctypes.POINTER(ctypes.c_longlong)])
     lib.model_regf.restype = None
     lib.model_regf.argtypes = ([ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp,
                                 _dp, _dp, _dp, ctypes.c_int, ctypes.c_int,
@@ -59,7 +65,50 @@ def model(tmp_path_factory):
                        m_flag, mode, band, BAND, p(acc), p(jrk), p(pot),
                        st.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)))
         return acc, jrk, pot, st
+    def run_tiles(c, h2, dtr, idx, m_flag):
+        """The shipped kernel's tile rule: j in tile order (light then heavy
+        bodies, each Morton-sorted), i = idx Morton-sorted in the j box and
+        cut into wavefronts of 64.  Returns the outputs in idx order."""
+        f = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        lo = c.x.min(0)
+        ext = (c.x.max(0) - lo).max()
+        mbar = np.mean(c.m)
+        heavy = (c.m > HEAVY * mbar).astype(np.uint64)
+        jo = np.argsort((heavy << np.uint64(30)) | morton_keys(c.x, lo, ext),
+                        kind="stable")
+        io = idx[np.argsort(morton_keys(c.x[idx], lo, ext), kind="stable")]
+        m, x, v = f(c.m[jo]), f(c.x[jo]), f(c.v[jo])
+        xi, vi, hi, di = f(c.x[io]), f(c.v[io]), f(h2[io]), f(dtr[io])
+        ni = len(io)
+        acc, jrk, pot = np.zeros((ni, 3)), np.zeros((ni, 3)), np.zeros(ni)
+        st = np.zeros(2, np.int64)
+        p = lambda a: a.ctypes.data_as(_dp)
+        lib.model_regf_tiles(c.n, p(m), p(x), p(v), ni, p(hi), p(di), p(xi),
+                             p(vi), m_flag, BAND, p(acc), p(jrk), p(pot),
+                             st.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)))
+        back = np.empty(ni, dtype=np.int64)
+        back[np.argsort(io, kind="stable")] = np.arange(ni)
+        pos = {int(g): k for k, g in enumerate(io)}
+        sel = np.array([pos[int(g)] for g in idx])
+        return acc[sel], jrk[sel], pot[sel], st
+    run.tiles = run_tiles
     return run
+
+
+def morton_keys(x, lo, ext):
+    """30-bit Morton codes of the points in the cube [lo, lo + ext)^3,
+    1024 cells a side (the engine's key)."""
+    q = np.clip(((x - lo) / ext * 1024).astype(np.int64), 0, 1023).astype(np.uint64)
+
+    def spread(v):
+        v = v & np.uint64(0x3ff)
+        v = (v | (v << np.uint64(16))) & np.uint64(0x030000ff)
+        v = (v | (v << np.uint64(8))) & np.uint64(0x0300f00f)
+        v = (v | (v << np.uint64(4))) & np.uint64(0x030c30c3)
+        v = (v | (v << np.uint64(2))) & np.uint64(0x09249249)
+        return v
+    return spread(q[:, 0]) | (spread(q[:, 1]) << np.uint64(1)) | \
+        (spread(q[:, 2]) << np.uint64(2))
 
 
 def morton_order(x):
@@ -165,3 +214,54 @@ def test_band_meets_bar_king_sample(model):
     (ea, ej, ep), frac = errors(model, c, h2, dtr, idx, 0, 1, BAND)
     assert max(ea, ej, ep) < FORCE_TOL / 2
     assert frac < 0.04
+
+
+def tile_errors(model, c, h2, dtr, idx, m_flag):
+    o = oracle.regf(c.m, c.x, c.v, h2[idx], dtr[idx], c.x[idx], c.v[idx],
+                    601, 550, m_flag)
+    acc, jrk, pot, st = model.tiles(c, h2, dtr, idx, m_flag)
+    ok = o[3][:, 0] >= 0
+    e = (rel(acc[ok], o[0][ok]).max(), rel(jrk[ok], o[1][ok]).max(),
+         rel(pot[ok], o[2][ok]).max())
+    near = st[0] / max(st[1], 1)
+    print(f"whole-tile kernel: acc {e[0]:.2e} jrk {e[1]:.2e} pot {e[2]:.2e}; "
+          f"near (wavefront, tile) pairs {near:.3f}")
+    return e, near
+
+
+def waves_sample(c, nw, seed):
+    """nw whole wavefronts of the Morton-sorted bodies (consecutive runs of
+    64 in Morton order), as the kernel forms them from a full block."""
+    lo = c.x.min(0)
+    ext = (c.x.max(0) - lo).max()
+    order = np.argsort(morton_keys(c.x, lo, ext), kind="stable")
+    w = np.random.default_rng(seed).choice(c.n // 64, nw, replace=False)
+    return np.sort(np.concatenate([order[64 * k:64 * k + 64] for k in w]))
+
+
+@pytest.mark.parametrize("m_flag", [0, 1])
+def test_whole_tile_kernel_config1_config2(model, m_flag):
+    """The shipped decision rule (FP32 only for tiles far from every lane of
+    a wavefront, whole near tiles in FP64) on configs 1 and 2: well inside
+    the bar, since its FP32 pairs are a subset of the per-pair model's."""
+    c, h2, dtr = config1(m_flag)
+    (ea, ej, ep), _ = tile_errors(model, c, h2, dtr, np.arange(c.n), m_flag)
+    assert max(ea, ej, ep) < FORCE_TOL / 2
+    c, h2, dtr = config2(m_flag)
+    idx = waves_sample(c, 96, 21 + m_flag)
+    (ea, ej, ep), _ = tile_errors(model, c, h2, dtr, idx, m_flag)
+    assert max(ea, ej, ep) < FORCE_TOL / 2
+
+
+def test_whole_tile_kernel_king_sample(model):
+    """King W0=6 N=131072, 24 whole wavefronts: errors far inside the bar
+    (the GPU measures jerk <= 4e-8 over all i), and about a third of the
+    (wavefront, tile) pairs near -- the FP64 share of the kernel's work."""
+    n = 131072
+    c = ics.king(n, 6.0, seed=203, m=ics.kroupa(n, seed=204))
+    rs, dtr = ics.neighbour_radii(c, k=48)
+    h2 = rs * rs
+    idx = waves_sample(c, 24, 31)
+    (ea, ej, ep), near = tile_errors(model, c, h2, dtr, idx, 0)
+    assert max(ea, ej, ep) < FORCE_TOL / 10
+    assert 0.1 < near < 0.45
