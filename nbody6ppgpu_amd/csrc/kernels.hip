@@ -10,7 +10,11 @@
 //                  max |v| and min positive m of the j set.
 //   k_predict      Hermite predictor of every j (xbpredall.f:18-26) -> JRec.
 //   k_jpset        scatter of corrected Hermite states into the predictor SoA.
-//   k_regf_pass1   pair sums + neighbour capture over one j slice per wave.
+//   k_regf_pass1   mixed kernel: pair sums + neighbour capture over one j
+//                  slice per workgroup of four wavefronts; tiles far from
+//                  every lane in FP32, near tiles in FP64 (_q: persistent
+//                  form taking tasks from per-XCD queues).
+//   k_regf_pass1_f64  the same in FP64 throughout (_q: persistent form).
 //   k_regf_pass2   per-i FP64 reduction of the slice partials, ordered list
 //                  compaction and nearest neighbour.
 //   k_pot_pass1/2  O(N^2) potential (gpupot.gpu.cu:32-57).
@@ -20,8 +24,6 @@
 #include <cfloat>
 #include <climits>
 #include <cmath>
-
-
 
 namespace gpunb {
 
