@@ -431,13 +431,12 @@ double pick_rounds(int ni) {
     const char *r = getenv("GPUNB_WAVE_ROUNDS");
     if (r && atof(r) > 0) return atof(r);
     // Resident-wave capacity multiples per launch, by block size: large
-    // blocks take 6 (smaller tasks even out the tail: N = 131072, 3.26 ms
-    // per launch at 4 against 3.51 at 2; with pass 2 overlapping the next
-    // chunk's pass 1, the whole step is about 1% faster at 6 or 8 than at
-    // 4), small ones fewer, since every extra slice is one more FP64
-    // partial per i to write and reduce (whole step, 1024 i: 0.187 ms at
-    // 1, 0.206 at 4; 128 i: 0.108 ms at 0.5, 0.121 at 4).
-    return std::min(6.0, std::max(0.5, ni / 1024.0));
+    // blocks take up to 12 (King N = 131072 in one launch, whole step:
+    // 16.96 ms at 6, 16.32 at 12, 16.74 at 24), small ones fewer, since
+    // every extra slice is one more FP64 partial per i to write and reduce
+    // (whole step, 1024 i: 0.187 ms at 1, 0.206 at 4; 128 i: 0.108 ms at
+    // 0.5, 0.121 at 4).
+    return std::min(12.0, std::max(0.5, ni / 1024.0));
 }
 
 // Pair arithmetic of a call.  Default: the mixed kernel (32) when the i run
@@ -487,11 +486,15 @@ bool pick_isort(int ni) {
     return ni >= 8192;
 }
 
-// i-particles per pass-1/pass-2 launch pair (GPUNB_CHUNK overrides).
+// i-particles per pass-1/pass-2 launch pair (GPUNB_CHUNK overrides).  Long
+// wavefronts pay their set-up and the end-of-task reduction once for more
+// tiles: King N = 131072 at 12 rounds, whole step 17.92 ms in chunks of
+// 32768, 16.99 of 65536, 16.32 in one launch of 131072.  The chunk bounds
+// the list scratch: 4 max(nnbmax, 1) bytes per i, two sets.
 int pick_chunk() {
     const char *c = getenv("GPUNB_CHUNK");
     if (c && atoi(c) >= 64) return atoi(c);
-    return 32768;
+    return 131072;
 }
 
 // Next free timing event (created on first use, reused after a read).
