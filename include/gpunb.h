@@ -169,7 +169,7 @@ int gpunb_regf_dev(int ni, const double *d_h2, const double *d_dtr,
  * needs (ni rounded up to 64) * (4 + 4 max(nnbmax, 1) + 64) bytes of it
  * (fewer j slices are used when the partials of the default decomposition
  * do not fit); a larger call is refused (-1, gpunb_last_error() says so).
- * The i run in the caller's order (no Morton sort).  Calls share the array:
+ * The i run in the caller's order (no Hilbert sort).  Calls share the array:
  * two calls that overlap on one GPU race (calls on the library stream run
  * in order). */
 int gpunb_regf_dev_noalloc(int ni, const double *d_h2, const double *d_dtr,
@@ -325,7 +325,7 @@ int gpunb_set_tuning(int group, double rounds);
  * tile-relative differences, with the tiles near an i evaluated in FP64;
  * 64 = FP64 pair arithmetic with the hardware 1/sqrt; 65 = FP64 with one
  * Newton step on 1/sqrt; 0 = default: 32 for calls whose i are run in
- * Morton order (gpunb_regf_ / gpunb_regf_dev with ni >= 8192), 65 for the
+ * Hilbert order (gpunb_regf_ / gpunb_regf_dev with ni >= 8192), 65 for the
  * others (scattered i leave few tiles far from every i of a wavefront).
  * Neighbour lists are identical in every mode.  Returns 0 or -1. */
 int gpunb_set_precision(int prec);

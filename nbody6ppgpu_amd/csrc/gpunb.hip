@@ -93,7 +93,7 @@ struct HostBuf {
     }
 };
 
-// The Morton-sorted view of a j set (see jprep.hip) and the scratch of its
+// The Hilbert-sorted view of a j set (see jprep.hip) and the scratch of its
 // preparation.
 struct SortedView {
     DevBuf<JRec> jps;
@@ -231,7 +231,7 @@ struct State {
     DevBuf<int> nbc[2];
     DevBuf<int> nbl[2];
     bool nbc_dirty[2] = {false, false};
-    // Morton order of the i of a call (large calls): keys, permutation, sort
+    // Hilbert order of the i of a call (large calls): keys, permutation, sort
     // storage and the i data gathered into that order.
     DevBuf<unsigned> ikey_in, ikey_out;
     DevBuf<int> ival_in, iperm;
@@ -440,7 +440,7 @@ double pick_rounds(int ni) {
 }
 
 // Pair arithmetic of a call.  Default: the mixed kernel (32) when the i run
-// in Morton order, since its FP32 far tiles need waves of nearby i; the
+// in Hilbert order, since its FP32 far tiles need waves of nearby i; the
 // FP64 kernel with one Newton step (65) otherwise: with i scattered over the
 // cluster nearly every tile is near some lane, and the mixed kernel would
 // run its FP64 loop almost everywhere with fewer resident waves (King
@@ -478,7 +478,7 @@ void ensure_counts(DevBuf<int> &b, size_t n) {
     launch_zero_int(S.st, b.p, (int)b.n);
 }
 
-// Calls of at least this many i are run in Morton order of the i
+// Calls of at least this many i are run in Hilbert order of the i
 // (GPUNB_ISORT=0 switches it off, GPUNB_ISORT=1 forces it on).
 bool pick_isort(int ni) {
     const char *e = getenv("GPUNB_ISORT");
@@ -1161,7 +1161,7 @@ struct Ens {
     hipGraphExec_t gexec = nullptr;
     std::vector<long long> gkey;
     long long last_pairs = 0;
-    // The i of every cluster in Morton order (for the mixed kernel): keys,
+    // The i of every cluster in Hilbert order (for the mixed kernel): keys,
     // permutation, gathered i data, and the clusters' i starts.
     DevBuf<unsigned long long> ikey_in, ikey_out;
     DevBuf<int> ival_in, iperm, d_i0;
@@ -1330,7 +1330,7 @@ void ens_regf_dev(int ncl, const int *ni, const double *h2, const double *dtr,
     REQUIRE(nnbmax <= kMaxNnbmax, "gpunb_ens_regf_dev: nnbmax above 4000 is not supported (a list row is sorted in LDS; the reference LMAX is at most 600)");
     set_device();
     const int group = pick_group();
-    // Every cluster's i run in Morton order (sorted here, before the graph
+    // Every cluster's i run in Hilbert order (sorted here, before the graph
     // replay below, so a replay reads this call's order): the mixed kernel
     // by default.
     const int prec = pick_prec(true);

@@ -32,7 +32,7 @@ struct alignas(32) PRec {
 };
 static_assert(sizeof(PRec) == 32, "PRec must be 32 bytes");
 
-// The j set in Morton order, as the FP32 pair loop reads it.  The sorted
+// The j set in Hilbert order, as the FP32 pair loop reads it.  The sorted
 // positions are grouped in tiles of kTile spatially close j; every j is held
 // as an FP32 offset from its tile's FP64 centre, so a pair difference is
 // (C_t - x_i) + o_j: the first term once per (i, tile) in FP64 rounded to
@@ -61,7 +61,7 @@ static_assert(sizeof(JTile) == 64, "JTile must be 64 bytes");
 
 // Per-cluster statistics of the j set, reduced in a fixed order (so every
 // run derives the same thresholds from them): bounding box of the positions
-// (the Morton quantisation box) and the mean mass.
+// (the curve quantisation box) and the mean mass.
 struct JClus {
     double lo[3];
     double hi[3];
@@ -100,8 +100,8 @@ struct PrepClus {
 // Buffers of the sorted j view (one per engine; the ensemble has its own).
 struct JView {
     const JRec *jp;     // original order (input of the preparation)
-    JRec *jps;          // FP64 records in Morton order
-    JRec32 *j32;        // FP32 tile-relative records in Morton order
+    JRec *jps;          // FP64 records in Hilbert order
+    JRec32 *j32;        // FP32 tile-relative records in Hilbert order
     JTile *tiles;
     int *jidx;          // sorted position -> original j (local to the cluster)
     JClus *jcl;         // per cluster
@@ -213,7 +213,7 @@ void launch_jpset(hipStream_t st, int n, const int *addr, const double *src,
                   double *t0, unsigned *err);
 void launch_jget(hipStream_t st, int n, const JRec *jp, double *x, double *v);
 
-// j preparation (jprep.hip): per-cluster statistics, Morton sort and the
+// j preparation (jprep.hip): per-cluster statistics, Hilbert sort and the
 // records of the sorted view, for ncl clusters whose ranges (device table pc)
 // tile [0, ntile_all * kTile).  nblk = sum over clusters of
 // prep_stat_blocks(nj).
@@ -232,7 +232,7 @@ struct PrepBufs {
 size_t jprep_tmp_bytes(int n, int ncl);
 hipError_t launch_jprep(hipStream_t st, const JView &V, const PrepClus *pc,
                         int ncl, int nblk, int ntile_all, const PrepBufs &B);
-// Morton order of the i of one call (jprep.hip): perm[p] = the original i
+// Hilbert order of the i of one call (jprep.hip): perm[p] = the original i
 // at sorted position p, and the i data gathered into that order.
 struct ISortBufs {
     unsigned *key_in, *key_out;                 // one cluster (32-bit keys)
@@ -246,7 +246,7 @@ size_t isort_tmp_bytes(int n);
 hipError_t launch_isort(hipStream_t st, const JClus *jcl, const double *xi,
                         const double *vi, const double *h2, const double *dtr,
                         int ni, const ISortBufs &B);
-// Ensemble: the i of every cluster in Morton order within the cluster (its
+// Ensemble: the i of every cluster in Hilbert order within the cluster (its
 // range [cl_i0[c], cl_i0[c + 1]) of the concatenated i kept), keyed in the
 // box of the cluster's j set.
 size_t isort_ens_tmp_bytes(int n, int ncl);

@@ -361,7 +361,7 @@ __device__ inline double f64_screen_thr(double h2d, double dtrd, double u0,
 
 // ---------------------------------------------------------------------------
 // Regular force, pass 1: one wavefront = 64 i-particles (one per lane) x one
-// contiguous slice of the Morton-sorted j set, tile by tile (64 j).  j
+// contiguous slice of the Hilbert-sorted j set, tile by tile (64 j).  j
 // records are wave-uniform, so they arrive through the scalar cache as SGPR
 // operands.  Each tile is run in one of two ways, chosen per wave:
 //
