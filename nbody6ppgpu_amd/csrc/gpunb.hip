@@ -114,10 +114,12 @@ struct SortedView {
         jps.ensure(cap_j);
         j32.ensure(cap_j);
         jidx.ensure(cap_j);
-        tiles.ensure(cap_j / kTile + 1);
+        // View tiles (kVTile j each), plus those of one spare tile.
+        const size_t nvt = cap_j / kVTile + kTile / kVTile;
+        tiles.ensure(nvt);
         jcl.ensure((size_t)std::max(ncl, 1));
         pc.ensure((size_t)std::max(ncl, 1));
-        launch_park_view(st, j32.p, (int)cap_j, tiles.p, (int)(cap_j / kTile + 1));
+        launch_park_view(st, j32.p, (int)cap_j, tiles.p, (int)nvt);
     }
     void release() {
         jps.release();

@@ -130,6 +130,13 @@ constexpr double kPadPos = 1.0e18;
 
 // Tile of j-particles between FP32 -> FP64 folds of the partial sums.
 constexpr int kTile = 64;
+// j per tile of the sorted view (FP32 offsets from a tile centre, the far
+// test of the mixed kernel): kTile / kVTile view tiles per tile.  32-j view
+// tiles leave fewer tiles near some lane (King N = 131072: 22.6% against
+// 26.9% of (wavefront, tile) pairs) but pay the per-tile set-up twice as
+// often: whole step 15.51 against 15.22 ms, so view tiles are whole tiles.
+constexpr int kVTile = 64;
+static_assert(kTile % kVTile == 0 && kVTile <= 64, "view tiles split a tile");
 
 // One independent cluster of a regular-force launch.  A launch covers ncl
 // clusters (the drop-in path uses one per i chunk, the ensemble path one per

@@ -37,6 +37,15 @@ __device__ inline double wmax(double v) {
     for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o));
     return v;
 }
+// The same within each kVTile-lane segment of the wavefront.
+__device__ inline double smin(double v) {
+    for (int o = kVTile / 2; o >= 1; o >>= 1) v = fmin(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ inline double smax(double v) {
+    for (int o = kVTile / 2; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    return v;
+}
 
 // FP32 value >= v (v finite, >= 0).
 __device__ inline float up32(double v) {
@@ -248,13 +257,18 @@ __global__ __launch_bounds__(64) void k_jtile(
     jps[p] = r;
     jidx[p] = local;
 
+    // The wavefront's lanes hold kTile / kVTile view tiles, one per
+    // kVTile-lane segment: every reduction below stays within a segment.
     double v[6] = {r.x, r.y, r.z, r.vx, r.vy, r.vz};
     double cen[6];
-    const bool any = __ballot(real) != 0;
+    const unsigned long long seg_mask =
+        kVTile >= 64 ? ~0ull : ((1ull << kVTile) - 1ull);
+    const bool any =
+        ((__ballot(real) >> (lane & ~(kVTile - 1))) & seg_mask) != 0;
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
-        const double lo = wmin(real ? v[k] : INFINITY);
-        const double hi = wmax(real ? v[k] : -INFINITY);
+        const double lo = smin(real ? v[k] : INFINITY);
+        const double hi = smax(real ? v[k] : -INFINITY);
         cen[k] = any ? 0.5 * lo + 0.5 * hi : 0.0;
     }
     JRec32 q;
@@ -286,10 +300,10 @@ __global__ __launch_bounds__(64) void k_jtile(
         q.w = 1.0f;
     }
     j32[p] = q;
-    e = wmax(e);
-    ev = wmax(ev);
-    w = wmax(real ? (double)q.w : 1.0);
-    if (lane == 0) {
+    e = smax(e);
+    ev = smax(ev);
+    w = smax(real ? (double)q.w : 1.0);
+    if ((lane & (kVTile - 1)) == 0) {
         JTile T;
         T.cx = cen[0];
         T.cy = cen[1];
@@ -302,7 +316,7 @@ __global__ __launch_bounds__(64) void k_jtile(
         T.wmax = (float)w;
         T.eb = up32((double)kTileBand * (double)kTileBand * (double)T.e *
                     (double)T.e * (1.0 + 1e-12));
-        tiles[t] = T;
+        tiles[t * (kTile / kVTile) + lane / kVTile] = T;
     }
 }
 

@@ -431,7 +431,7 @@ __device__ __forceinline__ void regf_pass1_task(
     const int nj = D.nj, ntile = D.ntile, ni = D.ni, n_iw = D.n_iw;
     const int nslice = D.nslice, cap = D.cap;
     const JRec32 *__restrict__ j32 = j32_all + D.j0;
-    const JTile *__restrict__ tiles = tile_all + D.j0 / kTile;
+    const JTile *__restrict__ tiles = tile_all + D.j0 / kVTile;
     const JRec *__restrict__ jps = jps_all + D.j0;
     const double mbar = jcl_all[D.jc].mbar;
     const double *__restrict__ xi = xi_all + 3 * (size_t)D.i0;
@@ -528,11 +528,13 @@ __device__ __forceinline__ void regf_pass1_task(
     // tile's branches the compiler would copy them to VGPRs every group).
     // The record arrays carry a spare tile past their end, so the last
     // look-ahead stays in bounds.
-    JTile Tn = tiles[tb];
-    for (int t = tb; t < te; ++t) {
+    // The wavefront's tiles [tb, te) in view tiles of kVTile j.
+    constexpr int kR = kTile / kVTile;
+    JTile Tn = tiles[tb * kR];
+    for (int t = tb * kR; t < te * kR; ++t) {
         const JTile T = Tn;
         Tn = tiles[t + 1];
-        const int jt = t * kTile;
+        const int jt = t * kVTile;
         const float Dx = (float)(T.cx - x0), Dy = (float)(T.cy - y0),
                     Dz = (float)(T.cz - z0);
         const float Vx = (float)(T.cu - u0), Vy = (float)(T.cv - w0),
@@ -574,7 +576,7 @@ __device__ __forceinline__ void regf_pass1_task(
                 Pa[k] = j32[jt + k];
                 opaque_sgpr(Pa[k]);
             }
-            for (int jg = 0; jg < kTile; jg += 2 * G) {
+            for (int jg = 0; jg < kVTile; jg += 2 * G) {
 #pragma unroll
                 for (int k = 0; k < G; ++k) Pb[k] = j32[jt + jg + G + k];
 #pragma unroll
@@ -639,7 +641,7 @@ __device__ __forceinline__ void regf_pass1_task(
                 Pa[k] = jps[jt + k];
                 opaque_sgpr(Pa[k]);
             }
-            for (int jg = 0; jg < kTile; jg += 2 * G) {
+            for (int jg = 0; jg < kVTile; jg += 2 * G) {
 #pragma unroll
                 for (int k = 0; k < G; ++k) Pb[k] = jps[jt + jg + G + k];
                 group64(Pa, jt + jg);

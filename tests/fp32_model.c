@@ -8,7 +8,7 @@
  *
  *   mode 0  pair differences x_j - x_i in FP64, rounded to FP32
  *   mode 1  tile-relative differences: j in the given order is cut into
- *           tiles of 64, C_t = midpoint of the tile's bounding box (positions
+ *           tiles of TILE, C_t = midpoint of the tile's bounding box (positions
  *           and velocities apart), dx = fl32(C_t - x_i) + fl32(x_j - C_t)
  *   mode 2  everything in FP64 except 1/r, rounded to FP32 (the floor any
  *           FP32 pair arithmetic has)
@@ -17,7 +17,7 @@
  * dx.dv with fused multiply-adds, 1/r = the correctly rounded FP32 value
  * moved by -1, 0 or +1 ulp (a deterministic hash of (i, j); the hardware
  * v_rsq_f32 is specified to 1 ulp), and the FP32 force terms accumulated in
- * FP32 and folded into FP64 every 64 j.  Near-band pairs -- min(r2, q) <
+ * FP32 and folded into FP64 every TILE j.  Near-band pairs -- min(r2, q) <
  * band * h2 * w_j with q = r2 + 2 dtr dx.dv, w_j = max(1, x^(2/3),
  * 1.4 x / K), x = m_j / mbar, band = K (times mbar for mass-weighted lists) --
  * and list neighbours are evaluated in FP64 as the oracle does; in mode 1
@@ -28,6 +28,9 @@
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
+
+/* j per tile of the engine's sorted view (kVTile). */
+#define TILE 64
 
 static uint32_t hash2(uint32_t a, uint32_t b) {
     uint32_t h = a * 0x9E3779B1u ^ (b + 0x7F4A7C15u + (a << 6) + (a >> 2));
@@ -64,7 +67,7 @@ void model_regf(int nj, const double *m, const double *xj, const double *vj,
     double mbar = 0.0;
     for (int j = 0; j < nj; j++) mbar += m[j];
     mbar = nj > 0 ? mbar / nj : 1.0;
-    int ntile = (nj + 63) / 64;
+    int ntile = (nj + TILE - 1) / TILE;
     double *cen = malloc(sizeof(double) * 7 * (size_t)(ntile > 0 ? ntile : 1));
     for (int t = 0; t < ntile; t++) {
         double lo[6], hi[6];
@@ -72,7 +75,7 @@ void model_regf(int nj, const double *m, const double *xj, const double *vj,
             lo[c] = INFINITY;
             hi[c] = -INFINITY;
         }
-        for (int j = 64 * t; j < 64 * t + 64 && j < nj; j++)
+        for (int j = TILE * t; j < TILE * t + TILE && j < nj; j++)
             for (int c = 0; c < 6; c++) {
                 double v = c < 3 ? xj[3 * (size_t)j + c] : vj[3 * (size_t)j + c - 3];
                 if (v < lo[c]) lo[c] = v;
@@ -81,7 +84,7 @@ void model_regf(int nj, const double *m, const double *xj, const double *vj,
         double *C = cen + 7 * t;
         for (int c = 0; c < 6; c++) C[c] = 0.5 * lo[c] + 0.5 * hi[c];
         double e = 0.0;
-        for (int j = 64 * t; j < 64 * t + 64 && j < nj; j++) {
+        for (int j = TILE * t; j < TILE * t + TILE && j < nj; j++) {
             double o2 = 0.0;
             for (int c = 0; c < 3; c++) {
                 const double o = (float)(xj[3 * (size_t)j + c] - C[c]);
@@ -120,7 +123,7 @@ void model_regf(int nj, const double *m, const double *xj, const double *vj,
             }
             float f[6];
             if (mode == 1) {
-                const double *C = cen + 7 * (j / 64);
+                const double *C = cen + 7 * (j / TILE);
                 for (int c = 0; c < 3; c++) {
                     f[c] = (float)(C[c] - x0[c]) + (float)(X[c] - C[c]);
                     f[3 + c] = (float)(C[3 + c] - v0[c]) + (float)(V[c] - C[3 + c]);
@@ -140,7 +143,7 @@ void model_regf(int nj, const double *m, const double *xj, const double *vj,
                 const float q = fmaf(dtr2x, rv, r2);
                 const float s = r2 < q ? r2 : q;
                 double T = Bh;
-                if (mode == 1 && cen[7 * (j / 64) + 6] > T) T = cen[7 * (j / 64) + 6];
+                if (mode == 1 && cen[7 * (j / TILE) + 6] > T) T = cen[7 * (j / TILE) + 6];
                 if ((double)s < T * w) fp64 = 1;
             }
             if (fp64 < 0) {
@@ -174,7 +177,7 @@ void model_regf(int nj, const double *m, const double *xj, const double *vj,
                 jy = fmaf(mr3, fmaf(al, f[1], f[4]), jy);
                 jz = fmaf(mr3, fmaf(al, f[2], f[5]), jz);
             }
-            if ((j & 63) == 63 || j == nj - 1) {
+            if ((j % TILE) == TILE - 1 || j == nj - 1) {
                 A[0] += ax; A[1] += ay; A[2] += az;
                 A[3] += jx; A[4] += jy; A[5] += jz; A[6] += ph;
                 ax = ay = az = jx = jy = jz = ph = 0;
@@ -193,9 +196,9 @@ void model_regf(int nj, const double *m, const double *xj, const double *vj,
 
 /*
  * model_regf_tiles -- the engine's mixed kernel as shipped: j in tile order
- * (the caller passes them light-Morton then heavy-Morton, cut into tiles of
- * 64), i in wavefronts of 64 (the caller passes them in Morton order; a
- * partial last wavefront has inactive lanes).  Per (wavefront, tile) every
+ * (the caller passes them light then heavy, each along the engine's curve,
+ * cut into tiles of TILE), i in wavefronts of 64 (the caller passes them in
+ * curve order; a partial last wavefront has inactive lanes).  Per (wavefront, tile) every
  * lane evaluates the far test in FP32 as the kernel does
  *   D = fl32(C_t - x_i), R = |D| (1 - 1e-5) - e_t,
  *   A = |dtr| (1 + 2u) (|V| + ev_t) (1 + 1e-5),
@@ -215,7 +218,7 @@ void model_regf_tiles(int nj, const double *m, const double *xj,
     double mbar = 0.0;
     for (int j = 0; j < nj; j++) mbar += m[j];
     mbar = nj > 0 ? mbar / nj : 1.0;
-    const int ntile = (nj + 63) / 64;
+    const int ntile = (nj + TILE - 1) / TILE;
     /* per tile: C[6], e, ev, wmax, eb (floats kept as doubles) */
     double *T = malloc(sizeof(double) * 10 * (size_t)(ntile > 0 ? ntile : 1));
     float *o = malloc(sizeof(float) * 6 * (size_t)(nj > 0 ? nj : 1));
@@ -225,7 +228,7 @@ void model_regf_tiles(int nj, const double *m, const double *xj,
             lo[c] = INFINITY;
             hi[c] = -INFINITY;
         }
-        for (int j = 64 * t; j < 64 * t + 64 && j < nj; j++)
+        for (int j = TILE * t; j < TILE * t + TILE && j < nj; j++)
             for (int c = 0; c < 6; c++) {
                 double v = c < 3 ? xj[3 * (size_t)j + c] : vj[3 * (size_t)j + c - 3];
                 if (v < lo[c]) lo[c] = v;
@@ -234,7 +237,7 @@ void model_regf_tiles(int nj, const double *m, const double *xj,
         double *C = T + 10 * t;
         for (int c = 0; c < 6; c++) C[c] = 0.5 * lo[c] + 0.5 * hi[c];
         double e = 0.0, ev = 0.0, wm = 1.0;
-        for (int j = 64 * t; j < 64 * t + 64 && j < nj; j++) {
+        for (int j = TILE * t; j < TILE * t + TILE && j < nj; j++) {
             double a = 0.0, b = 0.0;
             for (int c = 0; c < 3; c++) {
                 o[6 * (size_t)j + c] = (float)(xj[3 * (size_t)j + c] - C[c]);
@@ -309,7 +312,7 @@ void model_regf_tiles(int nj, const double *m, const double *xj,
             }
             s_all++;
             s_near += near;
-            const int j1 = 64 * t + 64 < nj ? 64 * t + 64 : nj;
+            const int j1 = TILE * t + TILE < nj ? TILE * t + TILE : nj;
             for (int l = 0; l < nl; l++) {
                 const int i = i0 + l;
                 const double *x0 = xi + 3 * (size_t)i, *v0 = vi + 3 * (size_t)i;
@@ -319,7 +322,7 @@ void model_regf_tiles(int nj, const double *m, const double *xj,
                     F[c] = (float)(C[c] - x0[c]);
                     F[3 + c] = (float)(C[3 + c] - v0[c]);
                 }
-                for (int j = 64 * t; j < j1; j++) {
+                for (int j = TILE * t; j < j1; j++) {
                     if (near) {
                         const double *X = xj + 3 * (size_t)j, *V = vj + 3 * (size_t)j;
                         double dd[6];
