@@ -134,7 +134,8 @@ constexpr int kTile = 64;
 // test of the mixed kernel): kTile / kVTile view tiles per tile.  32-j view
 // tiles leave fewer tiles near some lane (King N = 131072: 22.6% against
 // 26.9% of (wavefront, tile) pairs) but pay the per-tile set-up twice as
-// often: whole step 15.51 against 15.22 ms, so view tiles are whole tiles.
+// often: whole step 15.40-15.46 against 15.28-15.34 ms on one GPU, so view
+// tiles are whole tiles.
 constexpr int kVTile = 64;
 static_assert(kTile % kVTile == 0 && kVTile <= 64, "view tiles split a tile");
 
