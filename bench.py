@@ -693,13 +693,13 @@ def main():
     # (host/hermite_ac.cpp) integrates the same cluster through the engine's
     # ABI (device predictor + gpunb_regf_ + gpupot_) with ETAI 0.05 and
     # ETAR 0.1, once with the library's default kernel choice (the mixed
-    # kernel for blocks of at least 8192 i, run in Morton order; the FP64
+    # kernel for blocks of at least 8192 i, run in Hilbert order; the FP64
     # kernel for smaller, scattered blocks) and once with the FP64 kernel
     # for every block, side by side.
     hermite = []
     if not args.no_hermite:
         from nbody6ppgpu_amd import integrate
-        for prec, label in ((0, "default: mixed FP32/FP64 for Morton-sorted "
+        for prec, label in ((0, "default: mixed FP32/FP64 for Hilbert-sorted "
                                 "blocks >= 8192 i, FP64 below"),
                             (65, "FP64 (v_rsq_f64 + one Newton step)")):
             log(f"bench: Hermite/AC integration, ETAI 0.05 ETAR 0.1, {label}")
@@ -736,7 +736,7 @@ def main():
         eb = ens.upload(cl, nnb_e)
         pairs = float(sum(c[0].n ** 2 for c in cl))
         # The library default for the ensemble: the mixed kernel on every
-        # cluster's i in Morton order, pass 1 as a persistent launch.
+        # cluster's i in Hilbert order, pass 1 as a persistent launch.
         t_seq = ens.time_step(eb, nnb_e, "sequential", reps=3)
         t_bat = ens.time_step(eb, nnb_e, "batched", reps=5)
         t_gra = ens.time_step(eb, nnb_e, "graph", reps=5)
@@ -760,7 +760,7 @@ def main():
             "clusters": len(cl), "bodies": int(sum(c[0].n for c in cl)),
             "pairs_per_step": pairs,
             "kernel": "mixed (FP32 far tiles, FP64 near tiles), every "
-                      "cluster's i in Morton order",
+                      "cluster's i in Hilbert order",
             "sequential_pairs_per_s": pairs / t_seq,
             "batched_pairs_per_s": pairs / t_bat,
             "batched_one_workgroup_per_task_pairs_per_s": pairs / t_bat_plain,
@@ -817,8 +817,8 @@ def main():
                           "Newton steps)" % (args.prec - 64))
                          if args.prec >= 64 else
                          ("FP32 pair arithmetic on tile-relative "
-                          "differences in far tiles (j Morton-sorted in 64-j "
-                          "tiles with FP64 centres, i Morton-sorted); FP64 "
+                          "differences in far tiles (j Hilbert-sorted in 64-j "
+                          "tiles with FP64 centres, i Hilbert-sorted); FP64 "
                           "pair arithmetic for every tile within a "
                           "wavefront's near band (12 h2 w_j), the neighbour "
                           "predicate and the accumulation"),

@@ -163,7 +163,7 @@ def test_persistent_launch_matches_plain(prec):
     """The persistent pass 1 (per-XCD task queues, largest tasks first)
     computes every (i-wavefront, slice) partial exactly as the one-workgroup-
     per-task launch does: results bit-identical, in the default (mixed,
-    Morton-sorted i) and the FP64 kernel."""
+    Hilbert-sorted i) and the FP64 kernel."""
     cl = make_clusters([6000, 300, 2500, 64, 12000, 900], 801)
     nnbmax = 300
     ni = [c[0].n for c in cl]

@@ -44,7 +44,7 @@ def test_king_kroupa_16k_benchmark_steps():
 
 def test_king_kroupa_131072_one_time_unit():
     """Config 3's cluster size through the library's default kernel choice
-    (mixed kernel for Morton-sorted blocks of >= 8192 i, FP64 below)."""
+    (mixed kernel for Hilbert-sorted blocks of >= 8192 i, FP64 below)."""
     n = 131072
     c = ics.king(n, 6.0, seed=17, m=ics.kroupa(n, seed=18))
     rec = integrate.run(c.m, c.x, c.v, t_end=1.0, etai=0.03, etar=0.05,

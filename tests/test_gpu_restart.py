@@ -19,7 +19,7 @@ LMAX, NNBMAX = 301, 300
 
 
 def blocks(n, rng):
-    """The regular blocks of one step: every body (Morton-sorted mixed
+    """The regular blocks of one step: every body (Hilbert-sorted mixed
     kernel), 1000 scattered bodies (FP64 kernel), and 257 scattered bodies
     with mass-weighted lists."""
     return [(np.arange(n), 0), (np.sort(rng.choice(n, 1000, replace=False)), 0),
