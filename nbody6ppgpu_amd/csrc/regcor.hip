@@ -35,6 +35,76 @@ constexpr double kOne12 = 1.0 / 12.0;  // ONE12 (zero.f:116)
 constexpr int kMaxHalve = 2200;        // > 2098 halvings: DBL_MAX -> 0
 constexpr int kChunk = 8;              // list entries loaded per chunk
 constexpr int kWin = 4;                // look-ahead of the list merge
+// LDS rows of one workgroup (one wavefront, 64 bodies): 20 KB, so that the
+// eight workgroups a CU holds at N = 131072 stay resident together.
+constexpr int kStage = 5120;
+constexpr int kStageU = 8;             // rows whose loads are in flight together
+
+// Exclusive prefix sum of v over the 64 lanes; *total gets the full sum.
+__device__ inline int wave_excl_scan(int v, int lane, int *total) {
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    *total = __shfl(x, 63);
+    return x - v;
+}
+
+__device__ inline int *lane_ptr(int *p, int r) {
+    const unsigned long long a = (unsigned long long)p;
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)a, r);
+    const unsigned hi =
+        (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(a >> 32), r);
+    return (int *)(((unsigned long long)hi << 32) | lo);
+}
+
+// Row copies between HBM and LDS for the 64 lanes of a wavefront: lane r's
+// row (n_r ints at g_r in HBM, at s + b_r in LDS) is moved by the whole
+// wavefront, 64 consecutive entries per instruction, so one row costs a few
+// cache lines instead of one line per lane per entry.  kStageU rows are in
+// flight at a time.  Every lane must be active.
+template <bool kToLds>
+__device__ void move_rows(int *s, int b, int *g, int n, int lane) {
+    for (int r0 = 0; r0 < 64; r0 += kStageU) {
+        int v[kStageU];
+#pragma unroll
+        for (int u = 0; u < kStageU; ++u) {
+            const int nr = __builtin_amdgcn_readlane(n, r0 + u);
+            const int br = __builtin_amdgcn_readlane(b, r0 + u);
+            const int *gr = lane_ptr(g, r0 + u);
+            v[u] = lane < nr ? (kToLds ? gr[lane] : s[br + lane]) : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < kStageU; ++u) {
+            const int nr = __builtin_amdgcn_readlane(n, r0 + u);
+            const int br = __builtin_amdgcn_readlane(b, r0 + u);
+            int *gr = lane_ptr(g, r0 + u);
+            if (lane < nr) {
+                if (kToLds)
+                    s[br + lane] = v[u];
+                else
+                    gr[lane] = v[u];
+            }
+        }
+    }
+    // Rows longer than 64 entries: the rest, row by row.
+    unsigned long long rest = __ballot(n > 64);
+    while (rest) {
+        const int r = __builtin_ctzll(rest);
+        rest &= rest - 1;
+        const int nr = __builtin_amdgcn_readlane(n, r);
+        const int br = __builtin_amdgcn_readlane(b, r);
+        int *gr = lane_ptr(g, r);
+        for (int e = 64 + lane; e < nr; e += 64) {
+            if (kToLds)
+                s[br + e] = gr[e];
+            else
+                gr[e] = s[br + e];
+        }
+    }
+}
 
 // Pair force / derivative of neighbour record R on (xi, xidot), added
 // (sign > 0) to or subtracted from (f, fd) as regcor_gpu.F:349-357 and
@@ -79,92 +149,30 @@ __global__ void k_regcor_mark(RegcorLaunch L) {
     atomicAdd(&L.mark[I - 1], 1);
 }
 
-__global__ void k_regcor(RegcorLaunch L) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= L.ni) return;
+// NLIST / JJLIST scratch interleaved by body (entry k of body q at
+// (k - 1) ni + q): the lanes of a wavefront, which walk their lists nearly
+// in step, then share cache lines.  LIST entries through lrow.
+#define NL(k) nl[(size_t)((k) - 1) * ni_s]
+#define LI(k) lrow[(k) - 1]
+#define JJ(k) jj[(size_t)((k) - 1) * ni_s]
+
+// Everything after the list checks for one body (index I, lane q): radius,
+// list merge, irregular-force change, corrector, new list and steps.  LIST
+// is read and written through lrow: the body's LDS row (kLds, rsz entries)
+// or its row in HBM (grow).  Returns how many leading entries of the LDS
+// row were written (0 for none), for the copy back to LIST.
+template <bool kLds>
+__device__ __forceinline__ int regcor_body(const RegcorLaunch &L, int q, int I,
+                                           int nnb0, int nnb, int *lrow,
+                                           int *grow, int rsz) {
     const gpunb_regcor_par &P = L.par;
     const gpunb_regcor_state &S = L.st;
-    const int I = L.idx[q];
-    if (I < 1 || I > P.ntot) return;            // flagged by the pre-pass
     const int p = I - 1;
-    L.step_new[q] = S.step[p];
-    L.tnw_new[q] = S.timenw[p];
-    if (L.mark[p] != 1) {                       // listed twice: skip both
-        atomicOr(L.err, ERR_REGCOR);
-        return;
-    }
-    const int lmax = P.lmax;
-    int *LIST = S.list + (size_t)p * lmax;
-    // NLIST / JJLIST scratch interleaved by body (entry k of body q at
-    // (k - 1) ni + q): the lanes of a wavefront, which walk their lists
-    // nearly in step, then share cache lines.
     int *nl = L.nlist + q;
     int *jj = L.jj + q;
     const size_t ni_s = (size_t)L.ni;
-#define NL(k) nl[(size_t)((k) - 1) * ni_s]
-#define LI(k) LIST[(k) - 1]
-#define JJ(k) jj[(size_t)((k) - 1) * ni_s]
     auto rec = [&](int J) -> const JRec & { return L.jp[J - P.ifirst]; };
-    auto in_j = [&](int J) {
-        return J >= P.ifirst && J - P.ifirst < L.nj && J <= P.ntot;
-    };
-
-    // New list in util_gpu.F:103-110's form (1-based, the body dropped);
-    // both lists checked (members in the j set, strictly ascending) before
-    // anything is modified.  With sorted in-range lists every index of the
-    // list walks below stays inside its row.
-    const int cg = L.nnb[q];
-    const int og = L.off[q];
-    const int nnb0 = LI(1);
-    bool ok = cg >= 0 && og >= 0 && cg + 2 <= lmax &&
-              (long long)og + cg <= L.list_len && nnb0 >= 0 &&
-              nnb0 + 1 <= lmax && (I <= P.n || S.h != nullptr);
-    int nnb = 0;
-    // Both walks load kChunk entries at a time, so that one memory latency
-    // is paid per chunk rather than per entry.
-    if (ok) {
-        int prev = 0;
-        for (int k0 = 0; ok && k0 < cg; k0 += kChunk) {
-            int v[kChunk];
-#pragma unroll
-            for (int u = 0; u < kChunk; ++u)
-                v[u] = k0 + u < cg ? L.lst[og + k0 + u] : 0;
-#pragma unroll
-            for (int u = 0; u < kChunk; ++u) {
-                // Predicated rather than branched: after the first bad entry
-                // nothing more is written, and the body is rejected below.
-                const int J = v[u] + P.ifirst;
-                const bool live = ok && k0 + u < cg && J != I;
-                const bool good = in_j(J) && !(nnb > 0 && J <= prev) &&
-                                  nnb + 3 <= lmax;
-                if (live && !good) ok = false;
-                if (live && good) {
-                    NL(nnb + 2) = J;
-                    prev = J;
-                    ++nnb;
-                }
-            }
-        }
-        NL(1) = nnb;
-        prev = 0;
-        for (int l0 = 2; ok && l0 <= nnb0 + 1; l0 += kChunk) {
-            int v[kChunk];
-#pragma unroll
-            for (int u = 0; u < kChunk; ++u)
-                v[u] = l0 + u <= nnb0 + 1 ? LI(l0 + u) : 0;
-#pragma unroll
-            for (int u = 0; u < kChunk; ++u) {
-                const int l = l0 + u;
-                const bool live = l <= nnb0 + 1;
-                if (live && (!in_j(v[u]) || (l > 2 && v[u] <= prev))) ok = false;
-                if (live) prev = v[u];
-            }
-        }
-    }
-    if (!ok) {
-        atomicOr(L.err, ERR_REGCOR);
-        return;
-    }
+    int wmax = 0;
 
     double xi[3], xidot[3], freg[3], fdr[3];
 #pragma unroll
@@ -330,6 +338,7 @@ __global__ void k_regcor(RegcorLaunch L) {
             } else if (lg2 <= nnb) {
                 ++lg2;
                 LI(l2) = P.ntot + 1;       // l2 == l here
+                if (l2 > wmax) wmax = l2;
                 sent = true;
             } else {
                 break;
@@ -375,6 +384,7 @@ __global__ void k_regcor(RegcorLaunch L) {
                     ++nnb;
                     --nbloss;
                     LI(nnb0 + 1) = NL(1);
+                    if (nnb0 + 1 > wmax) wmax = nnb0 + 1;
                     ++c_smin;
                     pair_add(R, xi, xidot, false, freg, fdr);
                     if (k > nbloss) break;
@@ -459,9 +469,18 @@ __global__ void k_regcor(RegcorLaunch L) {
     }
     S.rs[p] = rs;
 
-    // New list (regcor_gpu.F:605-613).
+    // New list (regcor_gpu.F:605-613).  A list grown past the LDS row by
+    // the small-step retention goes straight to LIST in HBM: it covers every
+    // entry written to the LDS row before.
     if (nbloss + nbgain > 0) {
-        LI(1) = nnb;
+        int *dst = lrow;
+        if (kLds && nnb + 1 > rsz) {
+            dst = grow;
+            wmax = 0;
+        } else if (nnb + 1 > wmax) {
+            wmax = nnb + 1;
+        }
+        dst[0] = nnb;
         for (int l0 = 2; l0 <= nnb + 1; l0 += kChunk) {
             int v[kChunk];
 #pragma unroll
@@ -469,7 +488,7 @@ __global__ void k_regcor(RegcorLaunch L) {
                 v[u] = l0 + u <= nnb + 1 ? NL(l0 + u) : 0;
 #pragma unroll
             for (int u = 0; u < kChunk; ++u)
-                if (l0 + u <= nnb + 1) LI(l0 + u) = v[u];
+                if (l0 + u <= nnb + 1) dst[l0 + u - 1] = v[u];
         }
         c_flux = nbloss + nbgain;
     }
@@ -529,10 +548,146 @@ __global__ void k_regcor(RegcorLaunch L) {
     if (c_smin) atomicAdd(&L.cnt[1], c_smin);
     if (c_flux) atomicAdd(&L.cnt[2], c_flux);
     if (c_conv) atomicAdd(&L.cnt[3], c_conv);
+    return wmax;
+}
+
+// One wavefront of 64 bodies (lanes).  With kLds, each lane's new list is
+// copied into its LDS row and NLIST built from there; the same rows then
+// take the old LIST rows, which are checked, merged and rewritten in LDS
+// and copied back to LIST at the end.  Without kLds (the workgroup's rows
+// exceed kStage), the lanes walk their rows in HBM.
+template <bool kLds>
+__device__ __forceinline__ void regcor_wave(const RegcorLaunch &L, int *s_reg,
+                                            int lane, int q, bool lv, int I,
+                                            bool ok, int cg, int og,
+                                            int nnb0, int base, int rsz) {
+    const gpunb_regcor_par &P = L.par;
+    const gpunb_regcor_state &S = L.st;
+    const int lmax = P.lmax;
+    int *nl = L.nlist + q;
+    const size_t ni_s = (size_t)L.ni;
+    auto in_j = [&](int J) {
+        return J >= P.ifirst && J - P.ifirst < L.nj && J <= P.ntot;
+    };
+    int *grow = ok ? S.list + (size_t)(I - 1) * lmax : S.list;
+    const int *nsrc = ok ? L.lst + og : L.lst;
+    if (kLds) {
+        move_rows<true>(s_reg, base, const_cast<int *>(nsrc), ok ? cg : 0,
+                        lane);
+        __syncthreads();
+        nsrc = s_reg + base;
+    }
+    int nnb = 0;
+    // Both walks take kChunk entries at a time, so that one memory latency
+    // is paid per chunk rather than per entry.
+    if (ok) {
+        int prev = 0;
+        for (int k0 = 0; ok && k0 < cg; k0 += kChunk) {
+            int v[kChunk];
+#pragma unroll
+            for (int u = 0; u < kChunk; ++u)
+                v[u] = k0 + u < cg ? nsrc[k0 + u] : 0;
+#pragma unroll
+            for (int u = 0; u < kChunk; ++u) {
+                // Predicated rather than branched: after the first bad entry
+                // nothing more is written, and the body is rejected below.
+                const int J = v[u] + P.ifirst;
+                const bool live = ok && k0 + u < cg && J != I;
+                const bool good = in_j(J) && !(nnb > 0 && J <= prev) &&
+                                  nnb + 3 <= lmax;
+                if (live && !good) ok = false;
+                if (live && good) {
+                    NL(nnb + 2) = J;
+                    prev = J;
+                    ++nnb;
+                }
+            }
+        }
+        NL(1) = nnb;
+    }
+    int *lrow = grow;
+    if (kLds) {
+        __syncthreads();                      // every new list read
+        move_rows<true>(s_reg, base, grow, ok ? nnb0 + 1 : 0, lane);
+        __syncthreads();
+        lrow = s_reg + base;
+    }
+    if (ok) {
+        int prev = 0;
+        for (int l0 = 2; ok && l0 <= nnb0 + 1; l0 += kChunk) {
+            int v[kChunk];
+#pragma unroll
+            for (int u = 0; u < kChunk; ++u)
+                v[u] = l0 + u <= nnb0 + 1 ? LI(l0 + u) : 0;
+#pragma unroll
+            for (int u = 0; u < kChunk; ++u) {
+                const int l = l0 + u;
+                const bool live = l <= nnb0 + 1;
+                if (live && (!in_j(v[u]) || (l > 2 && v[u] <= prev))) ok = false;
+                if (live) prev = v[u];
+            }
+        }
+    }
+    if (lv && !ok) atomicOr(L.err, ERR_REGCOR);
+    int wmax = 0;
+    if (ok) wmax = regcor_body<kLds>(L, q, I, nnb0, nnb, lrow, grow, rsz);
+    if (kLds) {
+        __syncthreads();
+        move_rows<false>(s_reg, base, grow, wmax, lane);
+    }
+}
+
+__global__ __launch_bounds__(64) void k_regcor(RegcorLaunch L) {
+    __shared__ int s_reg[kStage];
+    const int lane = threadIdx.x;
+    const int q = blockIdx.x * 64 + lane;
+    const gpunb_regcor_par &P = L.par;
+    const gpunb_regcor_state &S = L.st;
+    // lv: index in range (others are flagged by the pre-pass) and listed
+    // once (a body listed twice is skipped in both places).
+    bool lv = false;
+    int I = 0;
+    if (q < L.ni) {
+        I = L.idx[q];
+        if (I >= 1 && I <= P.ntot) {
+            L.step_new[q] = S.step[I - 1];
+            L.tnw_new[q] = S.timenw[I - 1];
+            if (L.mark[I - 1] != 1)
+                atomicOr(L.err, ERR_REGCOR);
+            else
+                lv = true;
+        }
+    }
+    // New list in util_gpu.F:103-110's form (1-based, the body dropped);
+    // both lists checked (members in the j set, strictly ascending) before
+    // anything is modified.  With sorted in-range lists every index of the
+    // list walks stays inside its row.
+    const int lmax = P.lmax;
+    int cg = 0, og = 0, nnb0 = 0;
+    bool ok = false;
+    if (lv) {
+        cg = L.nnb[q];
+        og = L.off[q];
+        nnb0 = S.list[(size_t)(I - 1) * lmax];
+        ok = cg >= 0 && og >= 0 && cg + 2 <= lmax &&
+             (long long)og + cg <= L.list_len && nnb0 >= 0 &&
+             nnb0 + 1 <= lmax && (I <= P.n || S.h != nullptr);
+    }
+    // LDS row of a lane: its new list, then its old LIST row and the
+    // replacement (at most cg + 1 entries without the retention).
+    const int rsz = ok ? (cg + 1 > nnb0 + 1 ? cg + 1 : nnb0 + 1) : 0;
+    int total;
+    const int base = wave_excl_scan(rsz, lane, &total);
+    if (__builtin_amdgcn_readfirstlane(total) <= kStage)
+        regcor_wave<true>(L, s_reg, lane, q, lv, I, ok, cg, og, nnb0, base,
+                          rsz);
+    else
+        regcor_wave<false>(L, s_reg, lane, q, lv, I, ok, cg, og, nnb0, base,
+                           rsz);
+}
 #undef NL
 #undef LI
 #undef JJ
-}
 
 // New STEP / TIMENW after every body is corrected; marks cleared.
 __global__ void k_regcor_commit(RegcorLaunch L) {

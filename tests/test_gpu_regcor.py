@@ -110,6 +110,53 @@ def test_block_matches_oracle(variant):
     compare(got, want, f"n=4096 with c.m. bodies, {variant}")
 
 
+def test_lds_rows_hbm_rows_and_retention_growth():
+    """The library walks a workgroup's 64 lists in LDS when their rows fit
+    5120 ints (row of a body: max(new list incl. itself, old LIST) + 1) and
+    in HBM otherwise.  Lists of ~60 give both kinds of workgroup in one
+    call; bodies whose three nearest old neighbours have small steps while
+    two members are gained grow their list past that row by the small-step
+    retention (regcor_gpu.F:328-370), in both kinds."""
+    from scipy.spatial import cKDTree
+    n = 3072
+    par_kw, state, block = rc.build(n=n, seed=34, nnbopt=60, smin=1e-4)
+    x = block["xi"]
+    tree = cKDTree(x)
+    nnb, off, lst = block["nnb"].copy(), block["off"].copy(), block["lst"]
+    rows = [lst[off[i]:off[i] + nnb[i]] for i in range(n)]
+    grown = []
+    for i in range(5, n, 32):
+        old = [int(j) - 1 for j in state["list"][i, 1:1 + state["list"][i, 0]]]
+        if len(old) < 8:
+            continue
+        d = np.linalg.norm(x[old] - x[i], axis=1)
+        lost = [old[k] for k in np.argsort(d)[:3]]
+        _, cand = tree.query(x[i], k=len(old) + 12)
+        gain = [int(j) for j in cand if j != i and j not in old][:2]
+        state["step"][lost] = 1e-5
+        rows[i] = np.array(sorted(set(old) - set(lost) | set(gain) | {i}),
+                           np.int32)
+        grown.append(i)
+    pos = 0
+    for i, r in enumerate(rows):
+        nnb[i], off[i] = len(r), pos
+        pos += len(r)
+    block.update(nnb=nnb, off=off, lst=np.concatenate(rows))
+    want = oracle_run(par_kw, state, block, par_kw["lmax"])
+    # the cases this call holds, from its inputs and the oracle's lists
+    rsz = np.maximum(nnb + 1, state["list"][:, 0] + 1)
+    tot = rsz.reshape(-1, 64).sum(axis=1)
+    lds = tot <= 5120
+    past = np.nonzero(want[0]["list"][:, 0] + 1 > rsz)[0]
+    assert 0 < lds.sum() < len(tot), tot
+    assert np.any(lds[past // 64]) and np.any(~lds[past // 64]), past
+    assert want[3]["nbsmin"] >= len(past)
+    got = device_run(par_kw, state, block)
+    compare(got, want, f"n={n}, {int(lds.sum())} LDS and "
+            f"{int((~lds).sum())} HBM workgroups, {len(past)} lists grown "
+            "past their row")
+
+
 def test_subset_block_and_repeat():
     """A block that is a strided subset of the bodies, corrected twice in a
     row (the second call sees the first call's state)."""
