@@ -34,11 +34,16 @@ constexpr double kOne3 = 1.0 / 3.0;    // ONE3 (zero.f:113)
 constexpr double kOne12 = 1.0 / 12.0;  // ONE12 (zero.f:116)
 constexpr int kMaxHalve = 2200;        // > 2098 halvings: DBL_MAX -> 0
 constexpr int kChunk = 8;              // list entries loaded per chunk
-constexpr int kWin = 4;                // look-ahead of the list merge
+constexpr int kWin = 4;                // look-ahead of the list merge in LIST
+constexpr int kNWin = 16;              // NLIST entries per round of the merge
+// s_waitcnt vmcnt(0) (expcnt and lgkmcnt not waited on): every vector
+// memory operation of the wavefront complete.
+constexpr int kVmcnt0 = 0x0F70;
 // LDS rows of one workgroup (one wavefront, 64 bodies): 20 KB, so that the
 // eight workgroups a CU holds at N = 131072 stay resident together.
 constexpr int kStage = 5120;
-constexpr int kStageU = 8;             // rows whose loads are in flight together
+constexpr int kStageU = 16;            // rows whose loads are in flight together
+constexpr int kPairs = 4;              // list-change members loaded together
 
 // Exclusive prefix sum of v over the 64 lanes; *total gets the full sum.
 __device__ inline int wave_excl_scan(int v, int lane, int *total) {
@@ -306,57 +311,78 @@ __device__ __forceinline__ int regcor_body(const RegcorLaunch &L, int q, int I,
         int l = 2, lg = 2;
         NL(nnb + 2) = P.ntot + 1;
         NL(1) = LI(nnb0 + 1);
-        // wa[d] = LI(l + d), wb[d] = NL(lg + d): each step advances l and lg
-        // by 0 or 1, so the entries kWin ahead are loaded kWin steps before
-        // they are compared.  The loop writes LIST only at LI(l) (the
-        // sentinel, with l = NNB0 + 1 from then on) and never writes NLIST.
-        int wa[kWin], wb[kWin];
+        // wa[d] = LI(l + d): each step advances l and lg by 0 or 1, so the
+        // LIST entries kWin ahead are loaded kWin steps before they are
+        // compared.  NLIST is taken in rounds: kNWin entries from NL(lg) for
+        // every lane at once, one wait, then the walk runs on that window
+        // (wb[0] = NL(lg)) until it is used up, with no NLIST loads inside.
+        // The loop writes LIST only at LI(l) (the sentinel, with l = NNB0 +
+        // 1 from then on) and never writes NLIST.
+        int wa[kWin];
 #pragma unroll
-        for (int d = 0; d < kWin; ++d) {
-            wa[d] = l + d <= nnb0 + 1 ? LI(l + d) : 0;
-            wb[d] = lg + d <= nnb + 2 ? NL(lg + d) : 0;
-        }
-        for (;;) {
-            const int a = wa[0], b = wb[0];
-            int l2 = l, lg2 = lg;
-            if (a != b) {
-                if (a >= b) {
-                    ++nbgain;
-                    JJ(nnb0 + nbgain) = b;
-                    --l2;
-                } else {
-                    ++nbloss;
-                    JJ(nbloss) = a;
-                    if (S.step[a - 1] < P.smin) jmin = a;
-                    --lg2;
+        for (int d = 0; d < kWin; ++d) wa[d] = l + d <= nnb0 + 1 ? LI(l + d) : 0;
+        bool done = false;
+        while (!done) {
+            int wb[kNWin];
+#pragma unroll
+            for (int d = 0; d < kNWin; ++d)
+                wb[d] = lg + d <= nnb + 2 ? NL(lg + d) : 0;
+            __builtin_amdgcn_s_waitcnt(kVmcnt0);
+            int avail = kNWin;
+            while (avail > 0) {
+                const int a = wa[0], b = wb[0];
+                int l2 = l, lg2 = lg;
+                if (a != b) {
+                    if (a >= b) {
+                        ++nbgain;
+                        JJ(nnb0 + nbgain) = b;
+                        --l2;
+                    } else {
+                        ++nbloss;
+                        JJ(nbloss) = a;
+                        --lg2;
+                    }
                 }
-            }
-            bool sent = false;
-            if (l2 <= nnb0) {
-                ++l2;
-                ++lg2;
-            } else if (lg2 <= nnb) {
-                ++lg2;
-                LI(l2) = P.ntot + 1;       // l2 == l here
-                if (l2 > wmax) wmax = l2;
-                sent = true;
-            } else {
-                break;
-            }
-            if (l2 != l) {
+                bool sent = false;
+                if (l2 <= nnb0) {
+                    ++l2;
+                    ++lg2;
+                } else if (lg2 <= nnb) {
+                    ++lg2;
+                    LI(l2) = P.ntot + 1;       // l2 == l here
+                    if (l2 > wmax) wmax = l2;
+                    sent = true;
+                } else {
+                    done = true;
+                    break;
+                }
+                if (l2 != l) {
 #pragma unroll
-                for (int d = 0; d + 1 < kWin; ++d) wa[d] = wa[d + 1];
-                wa[kWin - 1] = l + kWin <= nnb0 + 1 ? LI(l + kWin) : 0;
-            } else if (sent) {
-                wa[0] = P.ntot + 1;
-            }
-            if (lg2 != lg) {
+                    for (int d = 0; d + 1 < kWin; ++d) wa[d] = wa[d + 1];
+                    wa[kWin - 1] = l + kWin <= nnb0 + 1 ? LI(l + kWin) : 0;
+                } else if (sent) {
+                    wa[0] = P.ntot + 1;
+                }
+                if (lg2 != lg) {
 #pragma unroll
-                for (int d = 0; d + 1 < kWin; ++d) wb[d] = wb[d + 1];
-                wb[kWin - 1] = lg + kWin <= nnb + 2 ? NL(lg + kWin) : 0;
+                    for (int d = 0; d + 1 < kNWin; ++d) wb[d] = wb[d + 1];
+                    --avail;
+                }
+                l = l2;
+                lg = lg2;
             }
-            l = l2;
-            lg = lg2;
+        }
+        for (int k0 = 1; k0 <= nbloss; k0 += kChunk) {
+            int v[kChunk];
+#pragma unroll
+            for (int u = 0; u < kChunk; ++u)
+                v[u] = k0 + u <= nbloss ? JJ(k0 + u) : 1;
+            double sv[kChunk];
+#pragma unroll
+            for (int u = 0; u < kChunk; ++u) sv[u] = S.step[v[u] - 1];
+#pragma unroll
+            for (int u = 0; u < kChunk; ++u)
+                if (k0 + u <= nbloss && sv[u] < P.smin) jmin = 1;
         }
         if (jmin != 0) {
             // Keep a lost neighbour with a small step inside 2 RS
@@ -398,19 +424,22 @@ __device__ __forceinline__ int regcor_body(const RegcorLaunch &L, int q, int I,
     }
 
     // Irregular force change of the list change (regcor_gpu.F:400-438),
-    // in list order; the next member's index and record are loaded while
-    // the current pair is evaluated.
+    // in list order, kPairs members at a time: their indices, then their
+    // records, are loaded together (past the end, the first member again).
     auto pair_run = [&](int first, int count, bool add) {
         if (count <= 0) return;
-        int jn = count > 1 ? JJ(first + 1) : 0;
-        JRec nx = rec(JJ(first));
-        for (int l = 0; l < count; ++l) {
-            const JRec cur = nx;
-            if (l + 1 < count) {
-                nx = rec(jn);
-                if (l + 2 < count) jn = JJ(first + l + 2);
-            }
-            pair_add(cur, xi, xidot, add, dfirr, dfd);
+        const int j0 = JJ(first);
+        for (int l0 = 0; l0 < count; l0 += kPairs) {
+            int jv[kPairs];
+#pragma unroll
+            for (int u = 0; u < kPairs; ++u)
+                jv[u] = l0 + u < count ? JJ(first + l0 + u) : j0;
+            JRec rv[kPairs];
+#pragma unroll
+            for (int u = 0; u < kPairs; ++u) rv[u] = rec(jv[u]);
+#pragma unroll
+            for (int u = 0; u < kPairs; ++u)
+                if (l0 + u < count) pair_add(rv[u], xi, xidot, add, dfirr, dfd);
         }
     };
     pair_run(1, nbloss, false);
