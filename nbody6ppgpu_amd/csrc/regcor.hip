@@ -187,6 +187,21 @@ __device__ __forceinline__ int regcor_body(const RegcorLaunch &L, int q, int I,
         freg[k] = L.freg[3 * (size_t)q + k];
         fdr[k] = L.fdr[3 * (size_t)q + k];
     }
+    // The body's own state, read before anything is stored: the state
+    // arrays may alias as far as the compiler knows, so a load placed after
+    // a store would wait for it.  No other lane writes this body's entries.
+    double fi0[3], fid0[3], fr0[3], frd0[3], x00[3], x0d0[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        fi0[k] = S.fi[3 * (size_t)p + k];
+        fid0[k] = S.fidot[3 * (size_t)p + k];
+        fr0[k] = S.fr[3 * (size_t)p + k];
+        frd0[k] = S.frdot[3 * (size_t)p + k];
+        x00[k] = S.x0[3 * (size_t)p + k];
+        x0d0[k] = S.x0dot[3 * (size_t)p + k];
+    }
+    const double step0 = S.step[p], stepr0 = S.stepr[p];
+    const double tnw0 = S.timenw[p], t00 = S.t0[p];
     double dfirr[3] = {0.0, 0.0, 0.0}, dfd[3] = {0.0, 0.0, 0.0};
     const double dtr = P.time - S.t0r[p];
     double ri2, rh2;
@@ -452,7 +467,7 @@ __device__ __forceinline__ int regcor_body(const RegcorLaunch &L, int q, int I,
     double dtsq12 = kOne12 * dtsq;
     double dtr13 = kOne3 * dtr;
     S.t0r[p] = P.time;
-    double stepi = S.step[p];
+    double stepi = step0;
     if (stepi < 5.0 * P.dtmin && dtr > 50.0 * stepi) {
         dtr13 = 0.0;
         dtsq12 = 0.0;
@@ -460,20 +475,20 @@ __device__ __forceinline__ int regcor_body(const RegcorLaunch &L, int q, int I,
     double fi[3], fid[3], d2r[3], d3r[3], xnd[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        fi[k] = S.fi[3 * (size_t)p + k];
-        fid[k] = S.fidot[3 * (size_t)p + k];
+        fi[k] = fi0[k];
+        fid[k] = fid0[k];
     }
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        const double dfr = S.fr[3 * (size_t)p + k] - freg[k] - dfirr[k];
+        const double dfr = fr0[k] - freg[k] - dfirr[k];
         const double fdr0 = fdr[k] + dfd[k];
-        const double frd = S.frdot[3 * (size_t)p + k];
+        const double frd = frd0[k];
         const double sum = frd + fdr0;
         const double at3 = 2.0 * dfr + dtr * sum;
         const double bt2 = -3.0 * dfr - dtr * (sum + frd);
         S.xn[3 * (size_t)p + k] =
-            S.x0[3 * (size_t)p + k] + ((double)0.6f * at3 + bt2) * dtsq12;
-        xnd[k] = S.x0dot[3 * (size_t)p + k] + (0.75 * at3 + bt2) * dtr13;
+            x00[k] + ((double)0.6f * at3 + bt2) * dtsq12;
+        xnd[k] = x0d0[k] + (0.75 * at3 + bt2) * dtr13;
         fi[k] = fi[k] + dfirr[k];
         fid[k] = fid[k] + dfd[k];
         if (k == 1) fid[0] = fid[0] - P.tidal4 * dfr;
@@ -510,13 +525,13 @@ __device__ __forceinline__ int regcor_body(const RegcorLaunch &L, int q, int I,
             wmax = nnb + 1;
         }
         dst[0] = nnb;
-        for (int l0 = 2; l0 <= nnb + 1; l0 += kChunk) {
-            int v[kChunk];
+        for (int l0 = 2; l0 <= nnb + 1; l0 += kNWin) {
+            int v[kNWin];
 #pragma unroll
-            for (int u = 0; u < kChunk; ++u)
+            for (int u = 0; u < kNWin; ++u)
                 v[u] = l0 + u <= nnb + 1 ? NL(l0 + u) : 0;
 #pragma unroll
-            for (int u = 0; u < kChunk; ++u)
+            for (int u = 0; u < kNWin; ++u)
                 if (l0 + u <= nnb + 1) dst[l0 + u - 1] = v[u];
         }
         c_flux = nbloss + nbgain;
@@ -531,7 +546,7 @@ __device__ __forceinline__ int regcor_body(const RegcorLaunch &L, int q, int I,
     w0 = P.etar * w0;
     double ttmp = sqrt(w0);
     const double dt0 = ttmp;
-    const double steprI = S.stepr[p];
+    const double steprI = stepr0;
     if (ttmp > 2.0 * steprI) {
         if (fmod(P.time, 2.0 * dtr) == 0.0) {
             ttmp = 2.0 * dtr < P.smax ? 2.0 * dtr : P.smax;
@@ -558,17 +573,17 @@ __device__ __forceinline__ int regcor_body(const RegcorLaunch &L, int q, int I,
     }
     if (P.smax < ttmp) ttmp = P.smax;
     S.stepr[p] = ttmp;
-    double tnw = S.timenw[p];
+    double tnw = tnw0;
     if (nnb0 == 0 && nnb >= 0) {
         stepi = 0.125 * stepi;
-        tnw = S.t0[p] + stepi;
+        tnw = t00 + stepi;
     }
     // Step halving (regcor_gpu.F:798-801), capped at kMaxHalve passes:
     // enough to take any finite double to zero, so the cap only ends the
     // loop where the reference's would not end (ttmp <= 0).
     for (int it = 0; it < kMaxHalve && ttmp < stepi; ++it) {
         stepi = 0.5 * stepi;
-        tnw = S.t0[p] + stepi;
+        tnw = t00 + stepi;
         ++c_conv;
     }
     L.step_new[q] = stepi;
