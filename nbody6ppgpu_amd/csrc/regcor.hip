@@ -34,13 +34,14 @@ constexpr double kOne3 = 1.0 / 3.0;    // ONE3 (zero.f:113)
 constexpr double kOne12 = 1.0 / 12.0;  // ONE12 (zero.f:116)
 constexpr int kMaxHalve = 2200;        // > 2098 halvings: DBL_MAX -> 0
 constexpr int kChunk = 8;              // list entries loaded per chunk
-constexpr int kWin = 4;                // look-ahead of the list merge in LIST
-constexpr int kNWin = 16;              // NLIST entries per round of the merge
+constexpr int kWin = 4;                // look-ahead of the list merge in NLIST
+constexpr int kNWin = 16;              // LIST entries per round of the merge
 // s_waitcnt vmcnt(0) (expcnt and lgkmcnt not waited on): every vector
 // memory operation of the wavefront complete.
 constexpr int kVmcnt0 = 0x0F70;
-// LDS rows of one workgroup (one wavefront, 64 bodies): 20 KB, so that the
-// eight workgroups a CU holds at N = 131072 stay resident together.
+// LDS rows of one workgroup (one wavefront, 64 bodies, an NLIST row each):
+// 20 KB, so that the eight workgroups a CU holds at N = 131072 stay
+// resident together.
 constexpr int kStage = 5120;
 constexpr int kStageU = 16;            // rows whose loads are in flight together
 constexpr int kPairs = 4;              // list-change members loaded together
@@ -154,30 +155,33 @@ __global__ void k_regcor_mark(RegcorLaunch L) {
     atomicAdd(&L.mark[I - 1], 1);
 }
 
-// NLIST / JJLIST scratch interleaved by body (entry k of body q at
+// NLIST entry k at nl[(k - 1) nls]: the body's LDS row, or the HBM scratch
+// NLG.  NLG and JJLIST are interleaved by body (entry k of body q at
 // (k - 1) ni + q): the lanes of a wavefront, which walk their lists nearly
-// in step, then share cache lines.  LIST entries through lrow.
-#define NL(k) nl[(size_t)((k) - 1) * ni_s]
+// in step, then share cache lines.  LIST entries through lrow (HBM).
+#define NL(k) nl[(size_t)((k) - 1) * nls]
+#define NLG(k) nlg[(size_t)((k) - 1) * ni_s]
 #define LI(k) lrow[(k) - 1]
 #define JJ(k) jj[(size_t)((k) - 1) * ni_s]
 
 // Everything after the list checks for one body (index I, lane q): radius,
-// list merge, irregular-force change, corrector, new list and steps.  LIST
-// is read and written through lrow: the body's LDS row (kLds, rsz entries)
-// or its row in HBM (grow).  Returns how many leading entries of the LDS
-// row were written (0 for none), for the copy back to LIST.
+// list merge, irregular-force change, corrector, new list and steps.  With
+// kLds, NLIST is the body's LDS row; the new list is left there for the
+// wavefront to write to LIST, and the count of its entries is returned (0:
+// LIST written here, or unchanged).
 template <bool kLds>
 __device__ __forceinline__ int regcor_body(const RegcorLaunch &L, int q, int I,
-                                           int nnb0, int nnb, int *lrow,
-                                           int *grow, int rsz) {
+                                           int nnb0, int nnb, int *nl,
+                                           size_t nls, int *lrow) {
     const gpunb_regcor_par &P = L.par;
     const gpunb_regcor_state &S = L.st;
     const int p = I - 1;
-    int *nl = L.nlist + q;
-    int *jj = L.jj + q;
     const size_t ni_s = (size_t)L.ni;
+    int *nlg = L.nlist + q;
+    int *jj = L.jj + q;
     auto rec = [&](int J) -> const JRec & { return L.jp[J - P.ifirst]; };
-    int wmax = 0;
+    int ncopy = 0;
+    bool nl_hbm = !kLds;
 
     double xi[3], xidot[3], freg[3], fdr[3];
 #pragma unroll
@@ -326,22 +330,22 @@ __device__ __forceinline__ int regcor_body(const RegcorLaunch &L, int q, int I,
         int l = 2, lg = 2;
         NL(nnb + 2) = P.ntot + 1;
         NL(1) = LI(nnb0 + 1);
-        // wa[d] = LI(l + d): each step advances l and lg by 0 or 1, so the
-        // LIST entries kWin ahead are loaded kWin steps before they are
-        // compared.  NLIST is taken in rounds: kNWin entries from NL(lg) for
+        // wb[d] = NL(lg + d): each step advances l and lg by 0 or 1, so the
+        // NLIST entries kWin ahead are read kWin steps before they are
+        // compared.  LIST is taken in rounds: kNWin entries from LI(l) for
         // every lane at once, one wait, then the walk runs on that window
-        // (wb[0] = NL(lg)) until it is used up, with no NLIST loads inside.
+        // (wa[0] = LI(l)) until it is used up, with no LIST loads inside.
         // The loop writes LIST only at LI(l) (the sentinel, with l = NNB0 +
         // 1 from then on) and never writes NLIST.
-        int wa[kWin];
+        int wb[kWin];
 #pragma unroll
-        for (int d = 0; d < kWin; ++d) wa[d] = l + d <= nnb0 + 1 ? LI(l + d) : 0;
+        for (int d = 0; d < kWin; ++d) wb[d] = lg + d <= nnb + 2 ? NL(lg + d) : 0;
         bool done = false;
         while (!done) {
-            int wb[kNWin];
+            int wa[kNWin];
 #pragma unroll
             for (int d = 0; d < kNWin; ++d)
-                wb[d] = lg + d <= nnb + 2 ? NL(lg + d) : 0;
+                wa[d] = l + d <= nnb0 + 1 ? LI(l + d) : 0;
             __builtin_amdgcn_s_waitcnt(kVmcnt0);
             int avail = kNWin;
             while (avail > 0) {
@@ -365,7 +369,6 @@ __device__ __forceinline__ int regcor_body(const RegcorLaunch &L, int q, int I,
                 } else if (lg2 <= nnb) {
                     ++lg2;
                     LI(l2) = P.ntot + 1;       // l2 == l here
-                    if (l2 > wmax) wmax = l2;
                     sent = true;
                 } else {
                     done = true;
@@ -373,15 +376,15 @@ __device__ __forceinline__ int regcor_body(const RegcorLaunch &L, int q, int I,
                 }
                 if (l2 != l) {
 #pragma unroll
-                    for (int d = 0; d + 1 < kWin; ++d) wa[d] = wa[d + 1];
-                    wa[kWin - 1] = l + kWin <= nnb0 + 1 ? LI(l + kWin) : 0;
+                    for (int d = 0; d + 1 < kNWin; ++d) wa[d] = wa[d + 1];
+                    --avail;
                 } else if (sent) {
                     wa[0] = P.ntot + 1;
                 }
                 if (lg2 != lg) {
 #pragma unroll
-                    for (int d = 0; d + 1 < kNWin; ++d) wb[d] = wb[d + 1];
-                    --avail;
+                    for (int d = 0; d + 1 < kWin; ++d) wb[d] = wb[d + 1];
+                    wb[kWin - 1] = lg + kWin <= nnb + 2 ? NL(lg + kWin) : 0;
                 }
                 l = l2;
                 lg = lg2;
@@ -401,7 +404,12 @@ __device__ __forceinline__ int regcor_body(const RegcorLaunch &L, int q, int I,
         }
         if (jmin != 0) {
             // Keep a lost neighbour with a small step inside 2 RS
-            // (regcor_gpu.F:328-370).
+            // (regcor_gpu.F:328-370), on NLIST in HBM (the LDS row has no
+            // room to grow).
+            if (kLds) {
+                for (int k = 1; k <= nnb + 1; ++k) NLG(k) = NL(k);
+                nl_hbm = true;
+            }
             int k = 1;
             for (;;) {
                 if (nnb > P.nnbmax || I > P.n) break;
@@ -416,16 +424,15 @@ __device__ __forceinline__ int regcor_body(const RegcorLaunch &L, int q, int I,
                 if (!skip) {
                     int ll = nnb + 1;
                     for (;;) {
-                        if (NL(ll) < J) break;
-                        NL(ll + 1) = NL(ll);
+                        if (NLG(ll) < J) break;
+                        NLG(ll + 1) = NLG(ll);
                         --ll;
                         if (!(ll > 1)) break;
                     }
-                    NL(ll + 1) = J;
+                    NLG(ll + 1) = J;
                     ++nnb;
                     --nbloss;
-                    LI(nnb0 + 1) = NL(1);
-                    if (nnb0 + 1 > wmax) wmax = nnb0 + 1;
+                    LI(nnb0 + 1) = NLG(1);
                     ++c_smin;
                     pair_add(R, xi, xidot, false, freg, fdr);
                     if (k > nbloss) break;
@@ -513,26 +520,25 @@ __device__ __forceinline__ int regcor_body(const RegcorLaunch &L, int q, int I,
     }
     S.rs[p] = rs;
 
-    // New list (regcor_gpu.F:605-613).  A list grown past the LDS row by
-    // the small-step retention goes straight to LIST in HBM: it covers every
-    // entry written to the LDS row before.
+    // New list (regcor_gpu.F:605-613): LIST(1) = NNB, LIST(2..NNB+1) =
+    // NLIST(2..NNB+1).  An NLIST row in LDS becomes that list by its first
+    // entry alone and is written to LIST by the wavefront; one in HBM is
+    // copied here.
     if (nbloss + nbgain > 0) {
-        int *dst = lrow;
-        if (kLds && nnb + 1 > rsz) {
-            dst = grow;
-            wmax = 0;
-        } else if (nnb + 1 > wmax) {
-            wmax = nnb + 1;
-        }
-        dst[0] = nnb;
-        for (int l0 = 2; l0 <= nnb + 1; l0 += kNWin) {
-            int v[kNWin];
+        if (kLds && !nl_hbm) {
+            NL(1) = nnb;
+            ncopy = nnb + 1;
+        } else {
+            LI(1) = nnb;
+            for (int l0 = 2; l0 <= nnb + 1; l0 += kNWin) {
+                int v[kNWin];
 #pragma unroll
-            for (int u = 0; u < kNWin; ++u)
-                v[u] = l0 + u <= nnb + 1 ? NL(l0 + u) : 0;
+                for (int u = 0; u < kNWin; ++u)
+                    v[u] = l0 + u <= nnb + 1 ? NLG(l0 + u) : 0;
 #pragma unroll
-            for (int u = 0; u < kNWin; ++u)
-                if (l0 + u <= nnb + 1) dst[l0 + u - 1] = v[u];
+                for (int u = 0; u < kNWin; ++u)
+                    if (l0 + u <= nnb + 1) LI(l0 + u) = v[u];
+            }
         }
         c_flux = nbloss + nbgain;
     }
@@ -592,38 +598,44 @@ __device__ __forceinline__ int regcor_body(const RegcorLaunch &L, int q, int I,
     if (c_smin) atomicAdd(&L.cnt[1], c_smin);
     if (c_flux) atomicAdd(&L.cnt[2], c_flux);
     if (c_conv) atomicAdd(&L.cnt[3], c_conv);
-    return wmax;
+    return ncopy;
 }
 
 // One wavefront of 64 bodies (lanes).  With kLds, each lane's new list is
-// copied into its LDS row and NLIST built from there; the same rows then
-// take the old LIST rows, which are checked, merged and rewritten in LDS
-// and copied back to LIST at the end.  Without kLds (the workgroup's rows
-// exceed kStage), the lanes walk their rows in HBM.
+// copied into its LDS row (cg + 2 entries at base, the list two entries in)
+// and NLIST built over it in place; the merge reads it there and the old
+// LIST row from HBM in rounds, and the new lists go back to LIST in
+// coalesced sweeps.  Without kLds (the workgroup's rows exceed kStage),
+// NLIST lives in the HBM scratch.
 template <bool kLds>
 __device__ __forceinline__ void regcor_wave(const RegcorLaunch &L, int *s_reg,
                                             int lane, int q, bool lv, int I,
                                             bool ok, int cg, int og,
-                                            int nnb0, int base, int rsz) {
+                                            int nnb0, int base) {
     const gpunb_regcor_par &P = L.par;
     const gpunb_regcor_state &S = L.st;
     const int lmax = P.lmax;
-    int *nl = L.nlist + q;
     const size_t ni_s = (size_t)L.ni;
     auto in_j = [&](int J) {
         return J >= P.ifirst && J - P.ifirst < L.nj && J <= P.ntot;
     };
     int *grow = ok ? S.list + (size_t)(I - 1) * lmax : S.list;
+    int *lrow = grow;
     const int *nsrc = ok ? L.lst + og : L.lst;
+    int *nl = L.nlist + q;
+    size_t nls = ni_s;
     if (kLds) {
-        move_rows<true>(s_reg, base, const_cast<int *>(nsrc), ok ? cg : 0,
-                        lane);
+        move_rows<true>(s_reg, base + 2, const_cast<int *>(nsrc),
+                        ok ? cg : 0, lane);
         __syncthreads();
-        nsrc = s_reg + base;
+        nsrc = s_reg + base + 2;
+        nl = s_reg + base;
+        nls = 1;
     }
     int nnb = 0;
-    // Both walks take kChunk entries at a time, so that one memory latency
-    // is paid per chunk rather than per entry.
+    // The new list kChunk entries at a time.  In LDS, NLIST(k) goes to
+    // nl[k - 1] while raw entry e sits at nl[e + 2]: a chunk writes below
+    // the next chunk's entries.
     if (ok) {
         int prev = 0;
         for (int k0 = 0; ok && k0 < cg; k0 += kChunk) {
@@ -649,22 +661,16 @@ __device__ __forceinline__ void regcor_wave(const RegcorLaunch &L, int *s_reg,
         }
         NL(1) = nnb;
     }
-    int *lrow = grow;
-    if (kLds) {
-        __syncthreads();                      // every new list read
-        move_rows<true>(s_reg, base, grow, ok ? nnb0 + 1 : 0, lane);
-        __syncthreads();
-        lrow = s_reg + base;
-    }
+    // The old LIST row from HBM, kNWin entries per wait.
     if (ok) {
         int prev = 0;
-        for (int l0 = 2; ok && l0 <= nnb0 + 1; l0 += kChunk) {
-            int v[kChunk];
+        for (int l0 = 2; ok && l0 <= nnb0 + 1; l0 += kNWin) {
+            int v[kNWin];
 #pragma unroll
-            for (int u = 0; u < kChunk; ++u)
+            for (int u = 0; u < kNWin; ++u)
                 v[u] = l0 + u <= nnb0 + 1 ? LI(l0 + u) : 0;
 #pragma unroll
-            for (int u = 0; u < kChunk; ++u) {
+            for (int u = 0; u < kNWin; ++u) {
                 const int l = l0 + u;
                 const bool live = l <= nnb0 + 1;
                 if (live && (!in_j(v[u]) || (l > 2 && v[u] <= prev))) ok = false;
@@ -673,11 +679,11 @@ __device__ __forceinline__ void regcor_wave(const RegcorLaunch &L, int *s_reg,
         }
     }
     if (lv && !ok) atomicOr(L.err, ERR_REGCOR);
-    int wmax = 0;
-    if (ok) wmax = regcor_body<kLds>(L, q, I, nnb0, nnb, lrow, grow, rsz);
+    int ncopy = 0;
+    if (ok) ncopy = regcor_body<kLds>(L, q, I, nnb0, nnb, nl, nls, lrow);
     if (kLds) {
         __syncthreads();
-        move_rows<false>(s_reg, base, grow, wmax, lane);
+        move_rows<false>(s_reg, base, grow, ncopy, lane);
     }
 }
 
@@ -717,23 +723,23 @@ __global__ __launch_bounds__(64) void k_regcor(RegcorLaunch L) {
              (long long)og + cg <= L.list_len && nnb0 >= 0 &&
              nnb0 + 1 <= lmax && (I <= P.n || S.h != nullptr);
     }
-    // LDS row of a lane: its new list, then its old LIST row and the
-    // replacement (at most cg + 1 entries without the retention).
-    const int rsz = ok ? (cg + 1 > nnb0 + 1 ? cg + 1 : nnb0 + 1) : 0;
+    // LDS row of a lane: NLIST, built over its new list (cg + 2 entries).
+    const int rsz = ok ? cg + 2 : 0;
     int total;
     const int base = wave_excl_scan(rsz, lane, &total);
     if (__builtin_amdgcn_readfirstlane(total) <= kStage)
-        regcor_wave<true>(L, s_reg, lane, q, lv, I, ok, cg, og, nnb0, base,
-                          rsz);
+        regcor_wave<true>(L, s_reg, lane, q, lv, I, ok, cg, og, nnb0, base);
     else
-        regcor_wave<false>(L, s_reg, lane, q, lv, I, ok, cg, og, nnb0, base,
-                           rsz);
+        regcor_wave<false>(L, s_reg, lane, q, lv, I, ok, cg, og, nnb0, base);
 }
 #undef NL
+#undef NLG
 #undef LI
 #undef JJ
 
-// New STEP / TIMENW after every body is corrected; marks cleared.
+// New STEP / TIMENW after every body is corrected, and the marks cleared.
+// A body listed twice reads its mark as 2, or as 0 once the other thread
+// has cleared it: skipped either way.
 __global__ void k_regcor_commit(RegcorLaunch L) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= L.ni) return;
@@ -744,14 +750,7 @@ __global__ void k_regcor_commit(RegcorLaunch L) {
         L.st.step[p] = L.step_new[q];
         L.st.timenw[p] = L.tnw_new[q];
     }
-}
-
-__global__ void k_regcor_unmark(RegcorLaunch L) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= L.ni) return;
-    const int I = L.idx[q];
-    if (I < 1 || I > L.par.ntot) return;
-    L.mark[I - 1] = 0;
+    L.mark[p] = 0;
 }
 
 }  // namespace
@@ -763,7 +762,6 @@ void launch_regcor(hipStream_t st, const RegcorLaunch &L) {
     hipLaunchKernelGGL(k_regcor, dim3((unsigned)((L.ni + 63) / 64)), dim3(64),
                        0, st, L);
     hipLaunchKernelGGL(k_regcor_commit, grid, block, 0, st, L);
-    hipLaunchKernelGGL(k_regcor_unmark, grid, block, 0, st, L);
 }
 
 }  // namespace gpunb

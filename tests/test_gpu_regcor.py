@@ -110,16 +110,17 @@ def test_block_matches_oracle(variant):
     compare(got, want, f"n=4096 with c.m. bodies, {variant}")
 
 
-def test_lds_rows_hbm_rows_and_retention_growth():
-    """The library walks a workgroup's 64 lists in LDS when their rows fit
-    5120 ints (row of a body: max(new list incl. itself, old LIST) + 1) and
-    in HBM otherwise.  Lists of ~60 give both kinds of workgroup in one
-    call; bodies whose three nearest old neighbours have small steps while
-    two members are gained grow their list past that row by the small-step
-    retention (regcor_gpu.F:328-370), in both kinds."""
+def test_lds_and_hbm_workgroups_and_retention():
+    """The library keeps a workgroup's 64 NLIST rows in LDS when they fit
+    5120 ints (row of a body: new list incl. itself + 2) and in HBM
+    otherwise.  Lists of ~64 give both kinds of workgroup in one call;
+    bodies whose three nearest old neighbours have small steps while two
+    members are gained keep those neighbours by the small-step retention
+    (regcor_gpu.F:328-370), which grows NLIST past its LDS row (it then
+    moves to HBM), in both kinds of workgroup."""
     from scipy.spatial import cKDTree
     n = 3072
-    par_kw, state, block = rc.build(n=n, seed=34, nnbopt=60, smin=1e-4)
+    par_kw, state, block = rc.build(n=n, seed=34, nnbopt=64, smin=1e-4)
     x = block["xi"]
     tree = cKDTree(x)
     nnb, off, lst = block["nnb"].copy(), block["off"].copy(), block["lst"]
@@ -144,17 +145,16 @@ def test_lds_rows_hbm_rows_and_retention_growth():
     block.update(nnb=nnb, off=off, lst=np.concatenate(rows))
     want = oracle_run(par_kw, state, block, par_kw["lmax"])
     # the cases this call holds, from its inputs and the oracle's lists
-    rsz = np.maximum(nnb + 1, state["list"][:, 0] + 1)
-    tot = rsz.reshape(-1, 64).sum(axis=1)
+    tot = (nnb + 2).reshape(-1, 64).sum(axis=1)
     lds = tot <= 5120
-    past = np.nonzero(want[0]["list"][:, 0] + 1 > rsz)[0]
+    grown = np.array(grown)
+    kept = grown[want[0]["list"][grown, 0] > nnb[grown] - 1]
     assert 0 < lds.sum() < len(tot), tot
-    assert np.any(lds[past // 64]) and np.any(~lds[past // 64]), past
-    assert want[3]["nbsmin"] >= len(past)
+    assert np.any(lds[kept // 64]) and np.any(~lds[kept // 64]), kept
     got = device_run(par_kw, state, block)
     compare(got, want, f"n={n}, {int(lds.sum())} LDS and "
-            f"{int((~lds).sum())} HBM workgroups, {len(past)} lists grown "
-            "past their row")
+            f"{int((~lds).sum())} HBM workgroups, {len(kept)} of the "
+            "modified lists grown by the retention")
 
 
 def test_subset_block_and_repeat():
