@@ -180,6 +180,9 @@ __device__ __forceinline__ int regcor_body(const RegcorLaunch &L, int q, int I,
     int *nlg = L.nlist + q;
     int *jj = L.jj + q;
     auto rec = [&](int J) -> const JRec & { return L.jp[J - P.ifirst]; };
+    auto in_j = [&](int J) {
+        return J >= P.ifirst && J - P.ifirst < L.nj && J <= P.ntot;
+    };
     int ncopy = 0;
     bool nl_hbm = !kLds;
 
@@ -340,13 +343,50 @@ __device__ __forceinline__ int regcor_body(const RegcorLaunch &L, int q, int I,
         int wb[kWin];
 #pragma unroll
         for (int d = 0; d < kWin; ++d) wb[d] = lg + d <= nnb + 2 ? NL(lg + d) : 0;
-        bool done = false;
+        // Every old member passes through a window once, in order, and is
+        // checked there (in the j set, strictly ascending): the walk ends
+        // only after the last member, and LIST is written only after that.
+        // Rows 16-byte aligned (LMAX a multiple of 4): five 16-byte loads
+        // per window instead of sixteen single ones.
+        const bool row16 = (((unsigned long long)L.st.list & 15) == 0) &&
+                           (P.lmax & 3) == 0;
+        bool done = false, bad = false;
+        int vprev = 0;
         while (!done) {
             int wa[kNWin];
+            if (row16) {
+                const int e0 = l - 1, o = e0 & 3;
+                const int4 *bp = (const int4 *)lrow + (e0 >> 2);
+                int e[kNWin + 4];
 #pragma unroll
-            for (int d = 0; d < kNWin; ++d)
-                wa[d] = l + d <= nnb0 + 1 ? LI(l + d) : 0;
+                for (int i = 0; i < kNWin / 4 + 1; ++i) {
+                    const int4 v = e0 - o + 4 * i <= nnb0 ? bp[i]
+                                                           : make_int4(0, 0, 0, 0);
+                    e[4 * i] = v.x;
+                    e[4 * i + 1] = v.y;
+                    e[4 * i + 2] = v.z;
+                    e[4 * i + 3] = v.w;
+                }
+#pragma unroll
+                for (int d = 0; d < kNWin; ++d) {
+                    const int x = o == 0 ? e[d] : o == 1 ? e[d + 1]
+                                : o == 2 ? e[d + 2] : e[d + 3];
+                    wa[d] = l + d <= nnb0 + 1 ? x : 0;
+                }
+            } else {
+#pragma unroll
+                for (int d = 0; d < kNWin; ++d)
+                    wa[d] = l + d <= nnb0 + 1 ? LI(l + d) : 0;
+            }
             __builtin_amdgcn_s_waitcnt(kVmcnt0);
+#pragma unroll
+            for (int d = 0; d < kNWin; ++d) {
+                if (l + d <= nnb0 + 1) {
+                    if (!in_j(wa[d]) || (l + d > 2 && wa[d] <= vprev)) bad = true;
+                    vprev = wa[d];
+                }
+            }
+            if (bad) break;
             int avail = kNWin;
             while (avail > 0) {
                 const int a = wa[0], b = wb[0];
@@ -389,6 +429,10 @@ __device__ __forceinline__ int regcor_body(const RegcorLaunch &L, int q, int I,
                 l = l2;
                 lg = lg2;
             }
+        }
+        if (bad) {
+            atomicOr(L.err, ERR_REGCOR);
+            return 0;
         }
         for (int k0 = 1; k0 <= nbloss; k0 += kChunk) {
             int v[kChunk];
@@ -660,23 +704,6 @@ __device__ __forceinline__ void regcor_wave(const RegcorLaunch &L, int *s_reg,
             }
         }
         NL(1) = nnb;
-    }
-    // The old LIST row from HBM, kNWin entries per wait.
-    if (ok) {
-        int prev = 0;
-        for (int l0 = 2; ok && l0 <= nnb0 + 1; l0 += kNWin) {
-            int v[kNWin];
-#pragma unroll
-            for (int u = 0; u < kNWin; ++u)
-                v[u] = l0 + u <= nnb0 + 1 ? LI(l0 + u) : 0;
-#pragma unroll
-            for (int u = 0; u < kNWin; ++u) {
-                const int l = l0 + u;
-                const bool live = l <= nnb0 + 1;
-                if (live && (!in_j(v[u]) || (l > 2 && v[u] <= prev))) ok = false;
-                if (live) prev = v[u];
-            }
-        }
     }
     if (lv && !ok) atomicOr(L.err, ERR_REGCOR);
     int ncopy = 0;

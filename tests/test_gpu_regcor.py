@@ -222,6 +222,22 @@ def test_invalid_bodies_are_reported_and_left_unchanged():
                                 block["vi"], block["freg"], block["fdr"],
                                 block["nnb"], block["off"], lst)
         assert np.array_equal(dev2.numpy()["fi"][20], state["fi"][20])
+        # an old LIST row out of order, and one with a member outside the j
+        # set: both reported, both bodies unchanged (LIST row included)
+        bad_state = rc.copy_state(state)
+        r30 = bad_state["list"][30]
+        if r30[0] >= 3:
+            r30[2], r30[3] = r30[3], r30[2]
+        bad_state["list"][31, 1 + bad_state["list"][31, 0] // 2] = n + 7
+        dev3 = regcor.DeviceState(bad_state)
+        with pytest.raises(gpunb.GpunbError, match="regular corrector"):
+            regcor.correct_host(par, dev3, block["idx"], block["xi"],
+                                block["vi"], block["freg"], block["fdr"],
+                                block["nnb"], block["off"], block["lst"])
+        out3 = dev3.numpy()
+        for i in (30, 31):
+            for k in ("fi", "fr", "xn", "rs", "step", "stepr", "list"):
+                assert np.array_equal(out3[k][i], bad_state[k][i]), (i, k)
         # host-side preconditions
         bad = regcor.make_par(**dict(par_kw, lmax=par_kw["nnbmax"] + 2))
         with pytest.raises(gpunb.GpunbError, match="lmax"):
@@ -248,6 +264,18 @@ def test_invalid_bodies_are_reported_and_left_unchanged():
             assert np.array_equal(after[k], before[k]), k
     finally:
         gpunb.gpunb_close()
+
+
+def test_list_rows_not_16_byte_aligned():
+    """LMAX not a multiple of 4: the old LIST rows are not 16-byte aligned,
+    so the merge loads them one entry at a time."""
+    par_kw, state, block = rc.build(n=1500, seed=35, lmax=603, smin=1e-4)
+    rng = np.random.default_rng(5)
+    state["step"][rng.choice(1500, 80, replace=False)] = 1e-5
+    want = oracle_run(par_kw, state, block, par_kw["lmax"])
+    got = device_run(par_kw, state, block)
+    assert want[3]["nbflux"] > 0
+    compare(got, want, "n=1500, lmax 603")
 
 
 def test_regf_to_corrector_chain_at_131072():
