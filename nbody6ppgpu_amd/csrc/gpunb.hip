@@ -952,8 +952,10 @@ void do_regf(int ni, const double *h2, const double *dtr, const double (*xi)[3],
                           n8 * sizeof(double), d_total);
     const double *dx = S.d_in.p, *dv = dx + 3 * (size_t)ni,
                  *dh = dx + 6 * (size_t)ni, *dt = dx + 7 * (size_t)ni;
+    // The nearest neighbour is searched only for gpunb_regfnn_.
     enqueue_regf(ni, dx, dv, dh, dt, d_acc, d_jrk, d_pot, nnbmax, m_flag,
-                 d_nnb, d_nnid, d_lst, d_off, d_total, list_cap);
+                 d_nnb, nnid ? d_nnid : nullptr, d_lst, d_off, d_total,
+                 list_cap);
 
     // Results back in one copy: the fixed-size outputs plus a speculative
     // prefix of the compact list (sized from the previous call); the rest,

@@ -186,6 +186,8 @@ __global__ __launch_bounds__(256) void k_predict(
     // with unit-stride loads (a thread reading its own three components
     // would touch each cache line in thirds).
     __shared__ double sx[768], sv[768], sf[768], sfd[768];
+    const double tj = j < nj ? t0[j] : 0.0;
+    const double mj = j < nj ? m[j] : 0.0;
     {
         const size_t base = 3 * (size_t)blockIdx.x * blockDim.x;
         const size_t lim = 3 * (size_t)nj;
@@ -203,7 +205,7 @@ __global__ __launch_bounds__(256) void k_predict(
     double vmag = 0.0, mpos = INFINITY;
     if (j < nj) {
         // xbpredall.f:18-26, evaluated exactly as written (no FMA).
-        const double s = t - t0[j];
+        const double s = t - tj;
         const double s1 = 1.5 * s;
         const double s2 = 2.0 * s;
         double xo[3], vo[3];
@@ -220,7 +222,7 @@ __global__ __launch_bounds__(256) void k_predict(
         r.vx = vo[0];
         r.vy = vo[1];
         r.vz = vo[2];
-        r.m = m[j];
+        r.m = mj;
         r.m32 = (float)r.m;
         r.pad = 0.0f;
         jp[j] = r;
@@ -1088,11 +1090,15 @@ __global__ __launch_bounds__(256) void k_regf_pass2(
                 const int p = ent[e];
                 const int j = jidx ? jidx[p] : p;
                 row[e] = j;
-                const JRec &J = jps[p];
-                const double r2 = sq3d(J.x - xa, J.y - ya, J.z - za);
-                if (r2 > 0.0 && (r2 < br2 || (r2 == br2 && j < bj))) {
-                    br2 = r2;
-                    bj = j;
+                // The nearest neighbour only when the caller asks for it:
+                // it costs a record gather per list entry.
+                if (nnid_all) {
+                    const JRec &J = jps[p];
+                    const double r2 = sq3d(J.x - xa, J.y - ya, J.z - za);
+                    if (r2 > 0.0 && (r2 < br2 || (r2 == br2 && j < bj))) {
+                        br2 = r2;
+                        bj = j;
+                    }
                 }
             }
             // Every lane of this wave reads the whole row below: the row's
