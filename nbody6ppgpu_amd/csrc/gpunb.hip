@@ -437,8 +437,9 @@ double pick_rounds(int ni) {
     // 16.96 ms at 6, 16.32 at 12, 16.74 at 24), small ones fewer, since
     // every extra slice is one more FP64 partial per i to write and reduce
     // (whole step, 1024 i: 0.187 ms at 1, 0.206 at 4; 128 i: 0.108 ms at
-    // 0.5, 0.121 at 4).
-    return std::min(12.0, std::max(0.5, ni / 1024.0));
+    // 0.5, 0.121 at 4).  1024 scattered i through gpunb_regf_: 5.05e11
+    // pairs/s at 0.75, 5.3e11 at 1, 5.2e11 at 1.5, 5.4e11 at 2.
+    return std::min(12.0, std::max(0.5, ni / 512.0));
 }
 
 // Pair arithmetic of a call.  Default: the mixed kernel (32) when the i run
