@@ -146,7 +146,7 @@ int gpunb_last_error(char *buf, int len);
  *   d_xi, d_vi : double[ni][3]     d_h2, d_dtr : double[ni]
  *   d_acc, d_jrk : double[ni][3]   d_pot : double[ni]
  *   d_nnb : int[ni]  (count, or -count on overflow, as list[i*lmax])
- *   d_nnid: int[ni]  (may be NULL)
+ *   d_nnid: int[ni]  (may be NULL: then the nearest neighbour is not searched)
  *   d_list: int[list_cap] compact neighbour lists; d_off[i] = start of i's
  *           list in d_list (-1 on overflow); d_total: int[1], set to the
  *           number of list entries written (the order of the per-i blocks in

@@ -16,7 +16,7 @@
 //                  form taking tasks from per-XCD queues).
 //   k_regf_pass1_f64  the same in FP64 throughout (_q: persistent form).
 //   k_regf_pass2   per-i FP64 reduction of the slice partials, ordered list
-//                  compaction and nearest neighbour.
+//                  compaction and (when asked for) the nearest neighbour.
 //   k_pot_pass1/2  O(N^2) potential (gpupot.gpu.cu:32-57).
 #include "gpunb_internal.h"
 
