@@ -3,8 +3,8 @@ host/hermite_ac.cpp calling gpunb_open_/jpset_/predict_/send_/regf_/gpupot_):
 energy conservation over one N-body time unit.
 
 ETAI = ETAR = 0.02 are the values of every sample deck the reference ships
-(samples/*.input, third line).  The bar is the north star's |dE/E| < 1e-5 per
-time unit: at N = 16384 with the benchmark's ETAI = 0.05 / ETAR = 0.1, and at
+(samples/*.input, third line).  The bar is |dE/E| < 1e-5 per time unit
+(the energy target set for the engine): at N = 16384 with the benchmark's ETAI = 0.05 / ETAR = 0.1, and at
 N = 131072 with ETAI = 0.03 / ETAR = 0.05 (at 0.05 / 0.1 the N = 131072
 cluster drifts to about 2.2e-5 with either pair kernel, FP64 included: the
 step sizes, not the pair arithmetic, set that drift).
