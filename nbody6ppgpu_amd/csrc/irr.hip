@@ -72,6 +72,11 @@ __device__ inline void predict(const IRec &r, double t, double *x, double *v) {
 }
 
 constexpr int kWavesPerBlock = 4;
+// Results of calls of at least kChunkMin bodies go to the host in kOutChunks
+// parts, so the host moves one part into the caller's arrays while the next
+// one crosses the host link.
+constexpr int kOutChunks = 4;
+constexpr int kChunkMin = 4 * 8192;
 
 // A particle predicted to the call's time: position, velocity, mass.
 struct alignas(64) PRec {
@@ -208,6 +213,26 @@ __global__ void k_irr_scatter_lists(int nrows, const int *__restrict__ hdr,
     for (int k = lane; k < cnt; k += 64) row[1 + k] = flat[off + k];
 }
 
+// Device -> mapped pinned host copy of three ranges of n16[b] 16-byte units,
+// blockIdx.y selecting the range: one part of a call's acceleration, jerk and
+// nearest-neighbour results.
+struct Ranges {
+    const uint4 *src[3];
+    uint4 *dst[3];
+    size_t n16[3];
+};
+
+__global__ __launch_bounds__(256) void k_irr_to_host(Ranges r) {
+    const int b = blockIdx.y;
+    const uint4 *__restrict__ src = r.src[b];
+    uint4 *__restrict__ dst = r.dst[b];
+    const size_t n16 = r.n16[b];
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n16;
+         k += stride)
+        dst[k] = src[k];
+}
+
 // Grow-only device / pinned host buffers.
 template <class T>
 struct DBuf {
@@ -261,6 +286,9 @@ struct State {
     HBuf<int> h_int;
     DBuf<IRec> d_rec;
     DBuf<int> d_int;
+    // results of a call: acc[ni_pad][3], jerk[ni_pad][3] (d_out, h_out) and
+    // the nearest neighbours[ni_pad] (d_nn, h_nn), ni_pad = ni rounded up to
+    // 64, so every part of them starts on a 16-byte boundary
     DBuf<double> d_out;
     DBuf<int> d_nn;
     HBuf<double> h_out;
@@ -271,6 +299,7 @@ struct State {
     // k_irr_force of the last call, timed on the stream
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_kernel_ms = 0.0;
+    hipEvent_t part[kOutChunks] = {};        // after the copy of part c
 };
 State I;
 
@@ -392,6 +421,8 @@ void do_open(int nmax, int lmax, int rank) {
     ICHK(hipStreamCreateWithFlags(&I.st, hipStreamNonBlocking));
     ICHK(hipEventCreate(&I.ev0));
     ICHK(hipEventCreate(&I.ev1));
+    for (int c = 0; c < kOutChunks; ++c)
+        ICHK(hipEventCreateWithFlags(&I.part[c], hipEventDisableTiming));
     I.last_kernel_ms = 0.0;
     I.nmax = nmax;
     I.lstride = lmax;
@@ -437,6 +468,10 @@ void do_close(int rank) {
     (void)hipEventDestroy(I.ev0);
     (void)hipEventDestroy(I.ev1);
     I.ev0 = I.ev1 = nullptr;
+    for (int c = 0; c < kOutChunks; ++c) {
+        (void)hipEventDestroy(I.part[c]);
+        I.part[c] = nullptr;
+    }
     (void)hipStreamDestroy(I.st);
     I.st = nullptr;
     I.open = false;
@@ -444,8 +479,9 @@ void do_close(int rank) {
 }
 
 // Upload the staged records and list rows, scatter them, then (ni > 0) the
-// force of addr[] at t; one host synchronisation at the end.  Layout of the
-// pinned int buffer: [jp slots (nj)] [row headers (3 nr)] [members] [i (ni)].
+// force of addr[] at t; the host waits only for each part of the results
+// (one part below kChunkMin bodies).  Layout of the pinned int buffer:
+// [jp slots (nj)] [row headers (3 nr)] [members] [i (ni)].
 void flush_and_force(double t, int ni, const int *addr, double *acc,
                      double *jrk, int *nnbid) {
     merge_stages();
@@ -498,10 +534,11 @@ void flush_and_force(double t, int ni, const int *addr, double *acc,
     I.ls_hdr.clear();
     I.ls_flat.clear();
     if (ni > 0) {
-        I.d_out.ensure((size_t)ni * 6);
-        I.d_nn.ensure((size_t)ni);
-        I.h_out.ensure((size_t)ni * 6);
-        I.h_nn.ensure((size_t)ni);
+        const size_t np = ((size_t)ni + 63) & ~(size_t)63;
+        I.d_out.ensure(np * 6);
+        I.d_nn.ensure(np);
+        I.h_out.ensure(np * 6);
+        I.h_nn.ensure(np);
         ICHK(hipEventRecord(I.ev0, I.st));
         // Predict every slot once when the call's pairs outnumber the slots
         // threefold: each member then costs a 64-byte read instead of a
@@ -517,31 +554,60 @@ void flush_and_force(double t, int ni, const int *addr, double *acc,
             hipLaunchKernelGGL(k_irr_force<true>, grid, dim3(64 * kWavesPerBlock),
                                0, I.st, ni, I.d_int.p + o_idx, t, I.jp.p,
                                I.pj.p, I.list.p, I.lstride, I.d_out.p,
-                               I.d_out.p + 3 * (size_t)ni, I.d_nn.p);
+                               I.d_out.p + 3 * np, I.d_nn.p);
         else
             hipLaunchKernelGGL(k_irr_force<false>, grid, dim3(64 * kWavesPerBlock),
                                0, I.st, ni, I.d_int.p + o_idx, t, I.jp.p,
                                (const PRec *)nullptr, I.list.p, I.lstride,
-                               I.d_out.p, I.d_out.p + 3 * (size_t)ni, I.d_nn.p);
+                               I.d_out.p, I.d_out.p + 3 * np, I.d_nn.p);
         ICHK(hipEventRecord(I.ev1, I.st));
-        ICHK(hipMemcpyAsync(I.h_out.p, I.d_out.p, (size_t)ni * 6 * sizeof(double),
-                            hipMemcpyDeviceToHost, I.st));
-        ICHK(hipMemcpyAsync(I.h_nn.p, I.d_nn.p, (size_t)ni * sizeof(int),
-                            hipMemcpyDeviceToHost, I.st));
-    }
-    ICHK(hipGetLastError());
-    // The pinned staging buffers are reused by the next call.
-    if (nint || nj) ICHK(hipStreamSynchronize(I.st));
-    if (ni > 0) {
+        // Part c: bodies q0 .. q0 + nq (q0 a multiple of 64), copied by one
+        // launch to the mapped h_out / h_nn; a byte count is rounded up to
+        // 16, which stays inside the padded arrays.
+        const int nchunk = ni >= kChunkMin ? kOutChunks : 1;
+        const int per = (int)((((size_t)ni + nchunk - 1) / nchunk + 63) & ~(size_t)63);
+        void *ho = nullptr, *hn = nullptr;
+        ICHK(hipHostGetDevicePointer(&ho, I.h_out.p, 0));
+        ICHK(hipHostGetDevicePointer(&hn, I.h_nn.p, 0));
+        for (int c = 0; c < nchunk; ++c) {
+            const size_t q0 = (size_t)c * per;
+            if (q0 >= (size_t)ni) break;
+            const size_t nq = std::min<size_t>(ni - q0, per);
+            Ranges r;
+            r.src[0] = (const uint4 *)(I.d_out.p + 3 * q0);
+            r.dst[0] = (uint4 *)((double *)ho + 3 * q0);
+            r.src[1] = (const uint4 *)(I.d_out.p + 3 * (np + q0));
+            r.dst[1] = (uint4 *)((double *)ho + 3 * (np + q0));
+            r.src[2] = (const uint4 *)(I.d_nn.p + q0);
+            r.dst[2] = (uint4 *)((int *)hn + q0);
+            r.n16[0] = r.n16[1] = (24 * nq + 15) / 16;
+            r.n16[2] = (4 * nq + 15) / 16;
+            const unsigned gx = (unsigned)std::min<size_t>(512, (r.n16[0] + 255) / 256);
+            hipLaunchKernelGGL(k_irr_to_host, dim3(gx, 3), dim3(256), 0, I.st, r);
+            ICHK(hipEventRecord(I.part[c], I.st));
+        }
+        ICHK(hipGetLastError());
+        // Each part into the caller's arrays once its copy is done (the
+        // uploads, the force and every earlier copy were before it on I.st).
+        for (int c = 0; c < nchunk; ++c) {
+            const size_t q0 = (size_t)c * per;
+            if (q0 >= (size_t)ni) break;
+            const size_t nq = std::min<size_t>(ni - q0, per);
+            ICHK(hipEventSynchronize(I.part[c]));
+            memcpy(acc + 3 * q0, I.h_out.p + 3 * q0, 3 * nq * sizeof(double));
+            memcpy(jrk + 3 * q0, I.h_out.p + 3 * (np + q0), 3 * nq * sizeof(double));
+            if (nnbid) memcpy(nnbid + q0, I.h_nn.p + q0, nq * sizeof(int));
+        }
         float ms = 0.0f;
         ICHK(hipEventElapsedTime(&ms, I.ev0, I.ev1));
         I.last_kernel_ms = ms;
-        memcpy(acc, I.h_out.p, (size_t)ni * 3 * sizeof(double));
-        memcpy(jrk, I.h_out.p + 3 * (size_t)ni, (size_t)ni * 3 * sizeof(double));
-        if (nnbid) memcpy(nnbid, I.h_nn.p, (size_t)ni * sizeof(int));
         I.ncall++;
         I.nsteps += ni;
         I.ninter += pairs;
+    } else {
+        ICHK(hipGetLastError());
+        // The pinned staging buffers are reused by the next call.
+        if (nint || nj) ICHK(hipStreamSynchronize(I.st));
     }
 }
 
