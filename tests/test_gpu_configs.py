@@ -221,9 +221,12 @@ def test_king_131072_invariants_all_particles():
 def test_chunked_launches_agree(monkeypatch):
     """One gpunb_regf_ of 70000 i split into launch pairs of 16384 (5
     chunks, pass 2 on the second stream over both scratch sets), 32768 (3
-    chunks, the default) and 131072 i (one chunk, one stream): identical
+    chunks) and 131072 i (one chunk, the default, one stream): identical
     lists and nearest neighbours; forces equal up to the FP64 summation
-    order of the slice partials, which the chunk size changes."""
+    order of the slice partials, which the chunk size changes.  The one-chunk
+    launch (1094 i-wavefronts in eight task blocks of 137, the last of 135,
+    its last wavefront 48 lanes) is also checked against the oracle on a
+    seeded sample of 256 i and the last 64 i of the input."""
     c = ics.king(70000, 6.0, seed=221, m=ics.kroupa(70000, seed=222))
     h2, dtr = setup(c, 48)
     lmax, nnbmax = 601, 550
@@ -239,6 +242,13 @@ def test_chunked_launches_agree(monkeypatch):
         gpunb.gpunb_close()
     ref = out["131072"]
     assert np.all(ref[3][:, 0] >= 1)
+    rng = np.random.default_rng(223)
+    sub = np.unique(np.concatenate([rng.choice(c.n, 256, replace=False),
+                                    np.arange(c.n - 64, c.n)]))
+    o = oracle.regf(c.m, c.x, c.v, h2[sub], dtr[sub], c.x[sub], c.v[sub],
+                    lmax, nnbmax, 0)
+    assert_parity(tuple(a[sub] for a in ref), o, label="King 70000 one launch",
+                  inputs=(c.m, c.x, c.v, c.x[sub], c.v[sub]))
     for chunk in ("16384", "32768"):
         g = out[chunk]
         assert np.array_equal(g[3], ref[3]), chunk
